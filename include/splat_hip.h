@@ -1,0 +1,184 @@
+/*
+ * splat_hip.h — C ABI of libsplat_hip.so, the MI355X (gfx950) Gaussian-splat
+ * rasterizer that plugs in behind splat-tui's `Renderer` trait as
+ * `--backend hip`.
+ *
+ * Every entry point returns 0 on success and a negative SP_E_* code on
+ * failure; sp_last_error() then holds a message for the calling thread.
+ * No exception, panic or abort crosses this boundary. Host buffers are owned
+ * by the caller; device buffers are owned by the handle. A handle is used by
+ * one thread at a time (the trait is `Send`, not `Sync`).
+ *
+ * Reference interfaces replaced (splat-tui, file:line):
+ *   sp_create / sp_destroy   render::create + CpuTiled::new
+ *                            (src/render/mod.rs:115-121, src/render/cpu.rs:82-91)
+ *   sp_render                Renderer::render (src/render/mod.rs:86-97),
+ *                            implemented as CpuTiled::render (src/render/cpu.rs:109-222)
+ *   sp_name                  Renderer::name (src/render/mod.rs:87)
+ *   sp_stats                 RenderStats (src/render/mod.rs:73-83)
+ *   sp_view                  camera::View (src/camera.rs:163-170)
+ *   sp_scene                 scene::Scene after finalize (src/scene.rs:11-28, :62-94)
+ */
+#ifndef SPLAT_HIP_H
+#define SPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants shared with the reference (src/render/mod.rs:23-33, cpu.rs:34) ---- */
+#define SP_TILE 16
+#define SP_MAX_DIM 16384u
+
+/* ---- error codes ---- */
+#define SP_OK 0
+#define SP_E_INVALID (-1)    /* bad argument (null pointer, inconsistent sizes) */
+#define SP_E_PIXEL_LIMIT (-2) /* width or height above SP_MAX_DIM ("pixel limit") */
+#define SP_E_DEVICE (-3)     /* HIP runtime error or no device */
+#define SP_E_NOMEM (-4)      /* device allocation failed / over the handle's budget */
+#define SP_E_CAPACITY (-5)   /* caller-supplied capacity too small (debug calls) */
+#define SP_E_UNSUPPORTED (-6)
+
+/* Storage precision of the per-splat parameters held on the device. */
+#define SP_STORE_F32 0
+#define SP_STORE_F16 1
+#define SP_STORE_BF16 2
+
+/*
+ * Activated scene parameters, laid out exactly as splat-tui's Scene vectors
+ * (glam Vec3 = 3 packed f32, Quat = x,y,z,w). `cov3d` is the upper triangle
+ * (xx, xy, xz, yy, yz, zz) that Scene::finalize computes (src/scene.rs:87-92,
+ * :170-178). `scales`/`rotations` may be NULL for SP_STORE_F32; the FP16/BF16
+ * storage variants and the gradient entry points need them.
+ */
+typedef struct sp_scene {
+    uint64_t n;               /* number of gaussians */
+    uint32_t sh_degree;       /* 0..3 */
+    uint32_t reserved;
+    const float *positions;   /* [n][3] */
+    const float *cov3d;       /* [n][6] */
+    const float *opacities;   /* [n] in [0,1] */
+    const float *colors;      /* [n][3] base colour (SH_C0*f_dc + 0.5) */
+    const float *sh_rest;     /* [n][(deg+1)^2-1][3], NULL when sh_degree == 0 */
+    const float *scales;      /* [n][3] or NULL */
+    const float *rotations;   /* [n][4] unit quaternions (x,y,z,w) or NULL */
+} sp_scene;
+
+/* camera::View (src/camera.rs:163-170): world->camera rows are right/down/forward. */
+typedef struct sp_view {
+    float eye[3];
+    float right[3];
+    float down[3];
+    float forward[3];
+    float fov_y;  /* radians */
+    float near_z; /* near plane distance (View::near) */
+} sp_view;
+
+/* RenderStats (src/render/mod.rs:73-83). `project_ms` covers projection, the
+ * per-splat depth sort and tile-pair emission; `sort_ms` the stable tile sort
+ * and tile ranges; `raster_ms` the blend. Device times from HIP events. */
+typedef struct sp_stats {
+    uint64_t visible;   /* gaussians that survived culling */
+    uint64_t overlaps;  /* (gaussian, 16x16 tile) pairs */
+    double project_ms;
+    double sort_ms;
+    double raster_ms;
+    double total_ms;
+} sp_stats;
+
+typedef struct sp_options {
+    int32_t device;            /* HIP device ordinal */
+    int32_t storage;           /* SP_STORE_* */
+    uint64_t initial_pairs;    /* initial pair capacity (0 = automatic) */
+    uint64_t max_device_bytes; /* 0 = unlimited; above it allocation fails with SP_E_NOMEM */
+    int32_t timing;            /* 1 = fill the *_ms fields of sp_stats from HIP events */
+    int32_t reserved;
+} sp_options;
+
+typedef struct sp_handle sp_handle;
+
+/* Defaults: device 0, f32 storage, automatic capacity, no limit, timing on. */
+void sp_default_options(sp_options *opt);
+
+/* Upload `scene` to the device and build the per-scene data (layout transform
+ * to structure-of-arrays, the bounding-box radius k = sqrt(2 ln(o/ALPHA_MIN)),
+ * as Prep::new does once per scene, src/render/cpu.rs:384-388).
+ * Replaces render::create(Backend::Hip, scene) (src/render/mod.rs:115-121). */
+int sp_create(const sp_scene *scene, const sp_options *opt, sp_handle **out);
+void sp_destroy(sp_handle *h);
+
+/* "hip" (Renderer::name, src/render/mod.rs:87). */
+const char *sp_name(const sp_handle *h);
+
+/* Renderer::render (src/render/mod.rs:89-96): render `view` at width x height
+ * over `background` into the caller's row-major RGB8 buffer of
+ * width*height*3 bytes. Zero width or height renders nothing and returns
+ * zeroed stats; a side above SP_MAX_DIM returns SP_E_PIXEL_LIMIT. */
+int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+              const float background[3], uint8_t *rgb_out, sp_stats *stats);
+
+/* As sp_render, but `rgb_out_device` is device memory (width*height*3 bytes)
+ * and no image is copied to the host. Returns when the frame is complete. */
+int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+                     const float background[3], uint8_t *rgb_out_device, sp_stats *stats);
+
+/* Last error message of the calling thread ("" if none). */
+const char *sp_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Inspection entry points (the oracle comparison surface). They have no direct
+ * counterpart in the reference's public API; they expose the intermediate
+ * arrays of CpuTiled::render (src/render/cpu.rs:131-200) in canonical form.
+ * ------------------------------------------------------------------------- */
+
+/* Per-splat projection of `view`: for every scene index i, valid[i] = 1 when
+ * the splat survives culling (render::project, src/render/mod.rs:201-272) and
+ * then proj[i*12 .. i*12+12] = x, y, conic a, b, c, r, g, b, opacity, depth,
+ * ext_x, ext_y, and bbox[i*4 .. +4] = the clamped pixel box x0, y0, x1, y1
+ * (src/render/cpu.rs:96-102). Arrays are n long. */
+int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+                     uint8_t *valid, float *proj, uint16_t *bbox);
+
+/* Sorted tile pairs of `view` in blend order: keys[p] = tile << 32 | depth
+ * bits, vals[p] = original scene index, tile_start[0..tiles] = bucket bounds
+ * (tiles = ceil(w/16)*ceil(h/16)). `cap` is the length of keys/vals; *n_pairs
+ * receives the total even when it exceeds cap (then SP_E_CAPACITY). */
+int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+                   uint64_t *keys, uint32_t *vals, uint32_t *tile_start, uint64_t cap,
+                   uint64_t *n_pairs);
+
+/* Work counters of the handle's last rendered view (no reference counterpart;
+ * the roofline numerators of the benchmark). */
+typedef struct sp_frame_info {
+    uint32_t width, height;
+    uint64_t tiles;          /* 16x16 tiles of the frame */
+    uint64_t visible;        /* = sp_stats.visible */
+    uint64_t pairs;          /* = sp_stats.overlaps */
+    uint64_t pairs_staged;   /* pair records the blend gathered before its tiles saturated */
+    uint64_t pair_capacity;  /* current pair buffer capacity */
+    uint64_t device_bytes;   /* device memory held by the handle */
+} sp_frame_info;
+
+int sp_last_frame_info(const sp_handle *h, sp_frame_info *info);
+
+/* ---------------------------------------------------------------------------
+ * Batched multi-view rendering (no reference counterpart; a new form of the
+ * same Renderer::render contract for many cameras at once).
+ * ------------------------------------------------------------------------- */
+
+/* Render n_views views at their own resolutions (wh[2*v], wh[2*v+1]) in one
+ * captured HIP graph. outs[v] receives width*height*3 bytes: host memory when
+ * outs_on_device == 0, device memory otherwise. stats may be NULL or n_views
+ * long. Images are identical to n_views calls of sp_render. */
+int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t n_views,
+                    const float background[3], uint8_t *const *outs, int32_t outs_on_device,
+                    sp_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPLAT_HIP_H */

@@ -1,0 +1,196 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of liboracle.so, the C++ restatement of splat-tui's `cpu` and
+`cpu-naive` rasterizers (oracle/splat_oracle.cpp). Only the tests, the smoke
+check and the CPU-baseline leg of bench.py use it, as the checker / CPU
+baseline; the HIP rasterizer never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from splat_amd import _native as N
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        sc = C.POINTER(N.SpScene)
+        vw = C.POINTER(N.SpView)
+        st = C.POINTER(N.SpStats)
+        L.or_cpu_create.restype = C.c_void_p
+        L.or_cpu_create.argtypes = [sc]
+        L.or_cpu_destroy.restype = None
+        L.or_cpu_destroy.argtypes = [C.c_void_p]
+        L.or_cpu_render.restype = C.c_int
+        L.or_cpu_render.argtypes = [C.c_void_p, vw, C.c_uint32, C.c_uint32, N.f32p, N.u8p, st, C.c_int, N.u64p]
+        L.or_cpu_pairs.restype = C.c_int
+        L.or_cpu_pairs.argtypes = [C.c_void_p, vw, C.c_uint32, C.c_uint32, N.u64p, N.u32p, N.u32p, C.c_uint64,
+                                   N.u64p, C.c_int]
+        L.or_naive_render.restype = C.c_int
+        L.or_naive_render.argtypes = [sc, vw, C.c_uint32, C.c_uint32, N.f32p, N.u8p, st]
+        L.or_project.restype = None
+        L.or_project.argtypes = [sc, vw, C.c_uint32, C.c_uint32, N.u8p, N.f32p, N.u16p]
+        L.or_exp_neg.restype = C.c_float
+        L.or_exp_neg.argtypes = [C.c_float]
+        L.or_to_u8.restype = C.c_uint8
+        L.or_to_u8.argtypes = [C.c_float]
+        L.or_sort_hi32.restype = None
+        L.or_sort_hi32.argtypes = [N.u64p, N.u64p, C.c_uint64]
+        L.or_fix_ties.restype = None
+        L.or_fix_ties.argtypes = [N.u64p, C.c_uint64, N.u32p]
+        L.or_eval_sh.restype = None
+        L.or_eval_sh.argtypes = [C.c_uint32, N.f32p, C.c_uint32, N.f32p, N.f32p]
+        L.or_bounds.restype = None
+        L.or_bounds.argtypes = [N.f32p, C.c_uint64, N.f32p, N.f32p]
+        L.or_spread3.restype = C.c_uint32
+        L.or_spread3.argtypes = [C.c_uint32]
+        L.or_orig_order.restype = None
+        L.or_orig_order.argtypes = [C.c_void_p, N.u32p]
+        _lib = L
+    return _lib
+
+
+def _bg(bg) -> np.ndarray:
+    return np.ascontiguousarray(bg, np.float32).reshape(3)
+
+
+class CpuTiled:
+    """Restatement of CpuTiled (src/render/cpu.rs): Morton-reordered scene, tile
+    buckets, per-tile radix sort + fix_ties, 4-lane blend with exp_neg."""
+
+    name = "cpu"
+
+    def __init__(self, scene, threads: int = 0):
+        cs, self._keep = scene.to_c()
+        self._scene = scene
+        self._h = lib().or_cpu_create(C.byref(cs))
+        self.threads = threads
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_cpu_destroy(self._h)
+            self._h = None
+
+    def render(self, view, width, height, background, work: bool = False):
+        img = np.zeros((height, width, 3), np.uint8)
+        st = N.SpStats()
+        wk = np.zeros(1, np.uint64)
+        rc = lib().or_cpu_render(self._h, C.byref(view), width, height, N.fptr(_bg(background)),
+                                 img.ctypes.data_as(N.u8p), C.byref(st), self.threads,
+                                 wk.ctypes.data_as(N.u64p) if work else None)
+        if rc:
+            raise RuntimeError("more than 2^32 gaussian/tile pairs")
+        return (img, st, int(wk[0])) if work else (img, st)
+
+    def pairs(self, view, width, height):
+        tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        ts = np.zeros(tiles + 1, np.uint32)
+        n = C.c_uint64(0)
+        cap = max(1, len(self._scene) * 64)
+        while True:
+            keys = np.zeros(cap, np.uint64)
+            vals = np.zeros(cap, np.uint32)
+            rc = lib().or_cpu_pairs(self._h, C.byref(view), width, height, keys.ctypes.data_as(N.u64p),
+                                    vals.ctypes.data_as(N.u32p), ts.ctypes.data_as(N.u32p), cap, C.byref(n),
+                                    self.threads)
+            if rc == -5:
+                cap = int(n.value)
+                continue
+            if rc:
+                raise RuntimeError("pair overflow")
+            k = int(n.value)
+            return keys[:k], vals[:k], ts
+
+    def orig(self) -> np.ndarray:
+        o = np.zeros(len(self._scene), np.uint32)
+        lib().or_orig_order(self._h, o.ctypes.data_as(N.u32p))
+        return o
+
+
+class CpuNaive:
+    """Restatement of CpuNaive (src/render/naive.rs): global stable depth sort,
+    per-splat blending with exp."""
+
+    name = "cpu-naive"
+
+    def __init__(self, scene):
+        self._cs, self._keep = scene.to_c()
+
+    def render(self, view, width, height, background):
+        img = np.zeros((height, width, 3), np.uint8)
+        st = N.SpStats()
+        lib().or_naive_render(C.byref(self._cs), C.byref(view), width, height, N.fptr(_bg(background)),
+                              img.ctypes.data_as(N.u8p), C.byref(st))
+        return img, st
+
+
+def project(scene, view, width, height):
+    """render::project for every splat: (valid bool[n], proj f32[n,12], bbox u16[n,4])."""
+    cs, keep = scene.to_c()
+    n = len(scene)
+    valid = np.zeros(n, np.uint8)
+    proj = np.zeros((n, 12), np.float32)
+    bbox = np.zeros((n, 4), np.uint16)
+    lib().or_project(C.byref(cs), C.byref(view), width, height, valid.ctypes.data_as(N.u8p), N.fptr(proj),
+                     bbox.ctypes.data_as(N.u16p))
+    del keep
+    return valid.astype(bool), proj, bbox
+
+
+def exp_neg(x: float) -> float:
+    return float(lib().or_exp_neg(float(x)))
+
+
+def to_u8(v: float) -> int:
+    return int(lib().or_to_u8(float(v)))
+
+
+def sort_hi32(keys: np.ndarray) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    tmp = np.zeros_like(k)
+    lib().or_sort_hi32(k.ctypes.data_as(N.u64p), tmp.ctypes.data_as(N.u64p), len(k))
+    return k
+
+
+def fix_ties(keys: np.ndarray, orig: np.ndarray) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    o = np.ascontiguousarray(orig, np.uint32)
+    lib().or_fix_ties(k.ctypes.data_as(N.u64p), len(k), o.ctypes.data_as(N.u32p))
+    return k
+
+
+def eval_sh(degree: int, rest: np.ndarray, direction) -> np.ndarray:
+    r = np.ascontiguousarray(rest, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(direction, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    lib().or_eval_sh(degree, N.fptr(r), len(r), N.fptr(d), N.fptr(out))
+    return out
+
+
+def bounds(positions: np.ndarray):
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    c = np.zeros(3, np.float32)
+    r = np.zeros(1, np.float32)
+    lib().or_bounds(N.fptr(p), len(p), N.fptr(c), N.fptr(r))
+    return c, float(r[0])
+
+
+def spread3(v: int) -> int:
+    return int(lib().or_spread3(v))

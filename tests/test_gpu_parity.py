@@ -1,0 +1,239 @@
+"""GPU parity of the HIP rasterizer against the oracle (C++ restatement of
+splat-tui's cpu / cpu-naive backends), through the C ABI.
+
+Bar: projection fields, key lists, sort order and tile ranges bit-exact; images
+byte-identical to the `cpu` restatement (its blend arithmetic is reproduced
+exactly); PSNR vs `cpu-naive` at least the reference's own 45 dB floor
+(tests/render.rs:45-76) — the two CPU backends themselves differ by that much.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import splat_amd as S
+from splat_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+BG = (0.1, 0.2, 0.3)
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _views(scene, n=3, up=S.UpAxis.NEG_Y):
+    cam = S.OrbitCamera.for_scene(scene, up)
+    out = []
+    for i in range(n):
+        cam.rotate(1.1, 0.0)
+        cam.pitch = 0.5 * math.sin(i)
+        cam.zoom(1.7 if i % 2 == 0 else 0.6)
+        out.append(cam.view())
+    return out
+
+
+SCENES = {}
+
+
+def scene(kind):
+    if kind not in SCENES:
+        if kind == "lattice":
+            SCENES[kind] = S.lattice()
+        elif kind == "random20k":
+            SCENES[kind] = S.random_scene(20000, seed=5, sh_degree=3)
+        elif kind == "random_sh1":
+            SCENES[kind] = S.random_scene(8000, seed=6, sh_degree=1)
+        elif kind == "random_sh2":
+            SCENES[kind] = S.random_scene(8000, seed=8, sh_degree=2)
+        else:
+            SCENES[kind] = S.test_scene(kind)
+    return SCENES[kind]
+
+
+KINDS = ["lattice", "cube", "shballs", "galaxy", "random20k", "random_sh1", "random_sh2"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_projection_bit_exact(kind, oracle):
+    s = scene(kind)
+    r = S.HipRenderer(s)
+    for vi, v in enumerate(_views(s)):
+        for (w, h) in [(320, 180), (37, 23)]:
+            case(f"project {kind} view {vi} {w}x{h}")
+            gv, gp, gb = r.debug_project(v, w, h)
+            ov, op, ob = oracle.project(s, v, w, h)
+            assert np.array_equal(gv, ov), f"{kind}: culling differs at {np.flatnonzero(gv != ov)[:5]}"
+            assert np.array_equal(gp[ov].view(np.uint32), op[ov].view(np.uint32)), f"{kind}: projected bits differ"
+            assert np.array_equal(gb[ov], ob[ov]), f"{kind}: bbox differs"
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pairs_bit_exact(kind, oracle):
+    s = scene(kind)
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    for vi, v in enumerate(_views(s)):
+        for (w, h) in [(320, 180), (333, 77), (16, 16)]:
+            case(f"pairs {kind} view {vi} {w}x{h}")
+            gk, gv, gt = r.debug_pairs(v, w, h)
+            ok, ov, ot = cpu.pairs(v, w, h)
+            assert np.array_equal(gt, ot), f"{kind} {w}x{h}: tile ranges differ"
+            assert len(gk) == len(ok)
+            assert np.array_equal(gk, ok), f"{kind} {w}x{h}: keys differ"
+            assert np.array_equal(gv, ov), f"{kind} {w}x{h}: blend order differs"
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_image_matches_cpu_bytes(kind, oracle):
+    s = scene(kind)
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    naive = oracle.CpuNaive(s)
+    for vi, v in enumerate(_views(s)):
+        for (w, h) in [(320, 180), (97, 53)]:
+            case(f"image {kind} view {vi} {w}x{h}")
+            img, st = r.render(v, w, h, BG)
+            ref, rst = cpu.render(v, w, h, BG)
+            assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
+            assert np.array_equal(img.pixels, ref), f"{kind} {w}x{h}: {np.count_nonzero(img.pixels != ref)} bytes differ"
+            nv, _ = naive.render(v, w, h, BG)
+            p = S.psnr(img.pixels, nv)
+            assert p >= 45.0, f"{kind} {w}x{h}: psnr vs cpu-naive {p}"
+
+
+def test_edge_sizes_and_up_axes(oracle):
+    """tests/render.rs:45-67: odd sizes x poses x all four up axes."""
+    for kind in ["cube", "galaxy", "lattice"]:
+        s = scene(kind)
+        r = S.HipRenderer(s)
+        cpu = oracle.CpuTiled(s)
+        for (w, h) in [(1, 1), (17, 9), (16, 16), (33, 31), (64, 48), (333, 77)]:
+            for yaw, pitch, up in [(0.37, 0.26, S.UpAxis.NEG_Y), (1.3, -0.5, S.UpAxis.NEG_Y), (4.0, 0.9, S.UpAxis.POS_Y),
+                                   (2.0, 0.1, S.UpAxis.POS_Z), (5.5, 0.3, S.UpAxis.NEG_Z)]:
+                case(f"edge {kind} {w}x{h} yaw {yaw} up {up.name}")
+                cam = S.OrbitCamera.for_scene(s, up)
+                cam.rotate(yaw, pitch - cam.pitch)
+                v = cam.view()
+                img, _ = r.render(v, w, h, BG)
+                ref, _ = cpu.render(v, w, h, BG)
+                assert np.array_equal(img.pixels, ref), f"{kind} {w}x{h} {up.name}"
+
+
+def test_render_semantics():
+    """Known answers of tests/render.rs:78-187 on the hip backend."""
+    RED, BLUE = (1.0, 0.0, 0.0), (0.0, 0.0, 1.0)
+    v = S.view_at((0.0, 0.0, 0.0))
+    case("semantics: single red splat")
+    s = S.scene_of([((0, 0, 5), 0.5, 1.0, RED)])
+    r = S.HipRenderer(s)
+    for (w, h) in [(64, 64), (65, 33), (200, 100)]:
+        f, _ = r.render(v, w, h, (0, 0, 0))
+        c = f.get(w // 2, h // 2)
+        assert c[0] > 200 and c[1] < 10 and c[2] < 10
+        for (x, y) in [(0, 0), (w - 1, 0), (0, h - 1), (w - 1, h - 1)]:
+            assert f.get(x, y) == (0, 0, 0)
+    case("semantics: background shows through")
+    f, _ = r.render(v, 64, 64, (0.0, 1.0, 0.0))
+    assert f.get(0, 0) == (0, 255, 0)
+    case("semantics: behind camera")
+    s = S.scene_of([((0, 0, -5), 0.5, 1.0, RED)])
+    f, st = S.HipRenderer(s).render(v, 40, 30, (0.5, 0.5, 0.5))
+    assert st.visible == 0 and np.all(f.pixels == 128)
+    case("semantics: off-screen and transparent")
+    s = S.scene_of([((500, 0, 5), 0.5, 1.0, RED), ((0, 0, 5), 0.5, 0.0, RED)])
+    f, _ = S.HipRenderer(s).render(v, 50, 50, (0.2, 0.4, 0.6))
+    assert np.all(f.pixels == f.pixels[0, 0])
+    case("semantics: empty scene")
+    f, st = S.HipRenderer(S.empty_scene()).render(v, 20, 10, (1, 1, 1))
+    assert np.all(f.pixels == 255) and st.visible == 0
+    case("semantics: occlusion independent of insertion order")
+    blue = ((0, 0, 4), 0.5, 1.0, BLUE)
+    red = ((0, 0, 8), 1.0, 1.0, RED)
+    for order in ([blue, red], [red, blue]):
+        f, _ = S.HipRenderer(S.scene_of(order)).render(v, 64, 64, (0, 0, 0))
+        c = f.get(32, 32)
+        assert c[2] > 200 and c[0] < 10
+    case("semantics: translucent front blends with back")
+    s = S.scene_of([((0, 0, 4), 0.5, 0.5, BLUE), ((0, 0, 8), 2.0, 1.0, RED)])
+    c = S.HipRenderer(s).render(v, 64, 64, (0, 0, 0))[0].get(32, 32)
+    assert c[0] > 40 and c[2] > 40
+    case("semantics: splat right of the axis lands in the right half")
+    s = S.scene_of([((1, 0, 5), 0.2, 1.0, RED)])
+    f, _ = S.HipRenderer(s).render(v, 100, 100, (0, 0, 0))
+    red_px = f.pixels[:, :, 0] > 100
+    assert red_px[:, 50:].sum() > 0 and red_px[:, :50].sum() == 0
+
+
+def test_deterministic_and_resize_safe():
+    """tests/render.rs:189-203."""
+    s = scene("cube")
+    r = S.HipRenderer(s)
+    v = S.OrbitCamera.for_scene(s).view()
+    case("determinism")
+    first, _ = r.render(v, 48, 32, (0, 0, 0))
+    small, _ = r.render(v, 7, 5, (0, 0, 0))
+    assert small.pixels.shape == (5, 7, 3)
+    again, _ = r.render(v, 48, 32, (0, 0, 0))
+    assert np.array_equal(first.pixels, again.pixels)
+
+
+def test_zero_size_and_pixel_limit():
+    s = scene("cube")
+    r = S.HipRenderer(s)
+    v = S.OrbitCamera.for_scene(s).view()
+    case("zero size")
+    for (w, h) in [(0, 0), (64, 0), (0, 64)]:
+        f, st = r.render(v, w, h, (0, 0, 0))
+        assert f.pixels.size == 0 and st.visible == 0
+    case("pixel limit")
+    with pytest.raises(ValueError, match="pixel limit"):
+        r.render(v, 16385, 4, (0, 0, 0))
+    st = N.SpStats()
+    import ctypes as C
+
+    bg = np.zeros(3, np.float32)
+    buf = np.zeros(16, np.uint8)
+    rc = N.hip().sp_render(r._h, C.byref(v), 16385, 1, N.fptr(bg), buf.ctypes.data_as(N.u8p), C.byref(st))
+    assert rc == N.SP_E_PIXEL_LIMIT and b"pixel limit" in N.hip().sp_last_error()
+
+
+def test_pair_capacity_growth(oracle):
+    """A tiny initial pair capacity forces the overflow -> grow -> re-render path
+    (the reference's gpu.min_cap hook, src/render/gpu.rs:1094-1096)."""
+    s = scene("galaxy")
+    v = _views(s)[0]
+    case("capacity growth")
+    r = S.HipRenderer(s, initial_pairs=64)
+    img, st = r.render(v, 320, 180, BG)
+    ref, rst = oracle.CpuTiled(s).render(v, 320, 180, BG)
+    assert st.overlaps == rst.overlaps and st.overlaps > 64
+    assert np.array_equal(img.pixels, ref)
+
+
+def test_device_budget_is_an_error():
+    case("device budget")
+    with pytest.raises(N.SplatError) as e:
+        S.HipRenderer(scene("cube"), max_device_bytes=1024)
+    assert e.value.code == N.SP_E_NOMEM
+
+
+def test_full_size_config_1m_1080p(oracle):
+    """1M random SH-3 splats at 1920x1080, first orbit frames: pairs, ranges and image
+    bit-exact against the CPU restatement."""
+    case("full size: generating 1M SH3 scene")
+    s = S.random_scene(1_000_000, seed=1, sh_degree=3)
+    views = S.orbit_views(s, 40, 20.0)
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    for fi in (0, 13):
+        case(f"full size frame {fi}: pairs")
+        gk, gv, gt = r.debug_pairs(views[fi], 1920, 1080)
+        ok, ov, ot = cpu.pairs(views[fi], 1920, 1080)
+        assert np.array_equal(gt, ot) and np.array_equal(gk, ok) and np.array_equal(gv, ov)
+        case(f"full size frame {fi}: image")
+        img, st = r.render(views[fi], 1920, 1080, (0, 0, 0))
+        ref, rst = cpu.render(views[fi], 1920, 1080, (0, 0, 0))
+        assert st.overlaps == rst.overlaps == len(ok)
+        assert np.array_equal(img.pixels, ref)
