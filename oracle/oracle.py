@@ -194,3 +194,44 @@ def bounds(positions: np.ndarray):
 
 def spread3(v: int) -> int:
     return int(lib().or_spread3(v))
+
+
+def _f64_sigs(L):
+    if getattr(L, "_f64", False):
+        return
+    sc = C.POINTER(N.SpScene)
+    vw = C.POINTER(N.SpView)
+    d64 = C.POINTER(C.c_double)
+    L.or_project_f64.restype = None
+    L.or_project_f64.argtypes = [sc, vw, C.c_uint32, C.c_uint32, N.u8p, d64, C.c_int]
+    L.or_render_f64.restype = None
+    L.or_render_f64.argtypes = [sc, vw, C.c_uint32, C.c_uint32, N.f32p, d64, N.u8p, C.c_int]
+    L._f64 = True
+
+
+def project_f64(scene, view, width, height, threads: int = 0):
+    """FP64 projection of every splat from the f32 parameters (covariance rebuilt from
+    scale/quat in double): (valid bool[n], proj f64[n,12])."""
+    L = lib()
+    _f64_sigs(L)
+    cs, keep = scene.to_c()
+    n = len(scene)
+    valid = np.zeros(n, np.uint8)
+    proj = np.zeros((n, 12), np.float64)
+    L.or_project_f64(C.byref(cs), C.byref(view), width, height, valid.ctypes.data_as(N.u8p),
+                     proj.ctypes.data_as(C.POINTER(C.c_double)), threads)
+    del keep
+    return valid.astype(bool), proj
+
+
+def render_f64(scene, view, width, height, background, threads: int = 0):
+    """FP64 render with cpu-naive semantics: (rgb f64[h,w,3] before quantisation, RGB8)."""
+    L = lib()
+    _f64_sigs(L)
+    cs, keep = scene.to_c()
+    rgb = np.zeros((height, width, 3), np.float64)
+    img = np.zeros((height, width, 3), np.uint8)
+    L.or_render_f64(C.byref(cs), C.byref(view), width, height, N.fptr(_bg(background)),
+                    rgb.ctypes.data_as(C.POINTER(C.c_double)), img.ctypes.data_as(N.u8p), threads)
+    del keep
+    return rgb, img

@@ -836,3 +836,203 @@ void or_orig_order(const or_cpu *h, uint32_t *orig) {
 }
 
 }  // extern "C"
+
+// ---- FP64 reference (the precision variant's error yardstick) -------------
+//
+// The same mathematics as render::project + sh::eval_rest + the cpu-naive
+// blend (src/render/mod.rs:201-278, src/sh.rs:39-64, src/render/naive.rs:45-93),
+// evaluated in double from the f32 scene parameters: the covariance from
+// scale/quat (src/scene.rs:170-178) when they are given, exp() instead of the
+// polynomial. Culling and the blend's thresholds are the reference's, so every
+// difference from it is arithmetic error of the path under test.
+
+namespace {
+
+struct P64 {
+    double x, y, conic[3], color[3], opacity, depth, ext_x, ext_y;
+    uint32_t bbox[4];
+};
+
+bool project64(const sp_scene *s, size_t i, const sp_view &v, uint32_t w, uint32_t h, P64 *o) {
+    const double op = s->opacities[i];
+    if (op < (double)ALPHA_MIN) return false;
+    double cov[6];
+    if (s->scales && s->rotations) {
+        const double sx = s->scales[3 * i], sy = s->scales[3 * i + 1], sz = s->scales[3 * i + 2];
+        const double qx = s->rotations[4 * i], qy = s->rotations[4 * i + 1], qz = s->rotations[4 * i + 2],
+                     qw = s->rotations[4 * i + 3];
+        const double R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy)},
+                                {2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx)},
+                                {2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)}};
+        const double sc[3] = {sx, sy, sz};
+        double S[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double acc = 0;
+                for (int k = 0; k < 3; ++k) acc += R[a][k] * sc[k] * sc[k] * R[b][k];
+                S[a][b] = acc;
+            }
+        cov[0] = S[0][0];
+        cov[1] = S[0][1];
+        cov[2] = S[0][2];
+        cov[3] = S[1][1];
+        cov[4] = S[1][2];
+        cov[5] = S[2][2];
+    } else {
+        for (int k = 0; k < 6; ++k) cov[k] = s->cov3d[6 * i + k];
+    }
+    const double d[3] = {s->positions[3 * i] - (double)v.eye[0], s->positions[3 * i + 1] - (double)v.eye[1],
+                         s->positions[3 * i + 2] - (double)v.eye[2]};
+    const double rot[3][3] = {{v.right[0], v.right[1], v.right[2]},
+                              {v.down[0], v.down[1], v.down[2]},
+                              {v.forward[0], v.forward[1], v.forward[2]}};
+    double t[3];
+    for (int r = 0; r < 3; ++r) t[r] = rot[r][0] * d[0] + rot[r][1] * d[1] + rot[r][2] * d[2];
+    if (!(t[2] > (double)v.near_z)) return false;
+    const double focal = (double)h * 0.5 / std::tan(0.5 * (double)v.fov_y);
+    const double x = focal * t[0] / t[2] + 0.5 * w, y = focal * t[1] / t[2] + 0.5 * h;
+    const double lx = 1.3 * (0.5 * w / focal), ly = 1.3 * (0.5 * h / focal);
+    const double tx = std::min(std::max(t[0] / t[2], -lx), lx), ty = std::min(std::max(t[1] / t[2], -ly), ly);
+    const double f = focal / t[2];
+    const double j0[3] = {f, 0, -f * tx}, j1[3] = {0, f, -f * ty};
+    double T0[3], T1[3];
+    for (int c = 0; c < 3; ++c) {
+        T0[c] = rot[0][c] * j0[0] + rot[1][c] * j0[1] + rot[2][c] * j0[2];
+        T1[c] = rot[0][c] * j1[0] + rot[1][c] * j1[1] + rot[2][c] * j1[2];
+    }
+    auto sig = [&](const double u[3], double r[3]) {
+        r[0] = cov[0] * u[0] + cov[1] * u[1] + cov[2] * u[2];
+        r[1] = cov[1] * u[0] + cov[3] * u[1] + cov[4] * u[2];
+        r[2] = cov[2] * u[0] + cov[4] * u[1] + cov[5] * u[2];
+    };
+    double s0[3], s1[3];
+    sig(T0, s0);
+    sig(T1, s1);
+    const double a = T0[0] * s0[0] + T0[1] * s0[1] + T0[2] * s0[2] + (double)DILATION;
+    const double b = T1[0] * s0[0] + T1[1] * s0[1] + T1[2] * s0[2];
+    const double cc = T1[0] * s1[0] + T1[1] * s1[1] + T1[2] * s1[2] + (double)DILATION;
+    const double det = a * cc - b * b;
+    if (!(det > 0)) return false;
+    const double k = std::sqrt(2.0 * std::log(op / (double)ALPHA_MIN));
+    const double ex = k * std::sqrt(a), ey = k * std::sqrt(cc);
+    if (x + ex <= 0 || x - ex >= w || y + ey <= 0 || y - ey >= h) return false;
+    double col[3] = {s->colors[3 * i], s->colors[3 * i + 1], s->colors[3 * i + 2]};
+    if (s->sh_degree > 0) {
+        const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double X = d[0] / len, Y = d[1] / len, Z = d[2] / len;
+        const uint32_t rest = (s->sh_degree + 1) * (s->sh_degree + 1) - 1;
+        const float *r = s->sh_rest + (size_t)3 * rest * i;
+        const double XX = X * X, YY = Y * Y, ZZ = Z * Z;
+        const double basis[15] = {-0.4886025119029199 * Y, 0.4886025119029199 * Z, -0.4886025119029199 * X,
+                                  1.0925484305920792 * X * Y, -1.0925484305920792 * Y * Z,
+                                  0.31539156525252005 * (2 * ZZ - XX - YY), -1.0925484305920792 * X * Z,
+                                  0.5462742152960396 * (XX - YY), -0.5900435899266435 * Y * (3 * XX - YY),
+                                  2.890611442640554 * X * Y * Z, -0.4570457994644658 * Y * (4 * ZZ - XX - YY),
+                                  0.3731763325901154 * Z * (2 * ZZ - 3 * XX - 3 * YY),
+                                  -0.4570457994644658 * X * (4 * ZZ - XX - YY), 1.445305721320277 * Z * (XX - YY),
+                                  -0.5900435899266435 * X * (XX - 3 * YY)};
+        for (uint32_t q = 0; q < rest; ++q)
+            for (int ch = 0; ch < 3; ++ch) col[ch] += basis[q] * (double)r[3 * q + ch];
+    }
+    o->x = x;
+    o->y = y;
+    o->conic[0] = cc / det;
+    o->conic[1] = -b / det;
+    o->conic[2] = a / det;
+    for (int c = 0; c < 3; ++c) o->color[c] = std::max(col[c], 0.0);
+    o->opacity = op;
+    o->depth = t[2];
+    o->ext_x = ex;
+    o->ext_y = ey;
+    const double fx0 = std::max(std::floor(x - ex), 0.0), fy0 = std::max(std::floor(y - ey), 0.0);
+    const uint32_t x1 = (uint32_t)std::min(std::max(std::ceil(x + ex), 0.0), (double)w);
+    const uint32_t y1 = (uint32_t)std::min(std::max(std::ceil(y + ey), 0.0), (double)h);
+    o->bbox[0] = std::min((uint32_t)std::min(fx0, 4.0e9), x1);
+    o->bbox[1] = std::min((uint32_t)std::min(fy0, 4.0e9), y1);
+    o->bbox[2] = x1;
+    o->bbox[3] = y1;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 projection of every splat: valid[i], proj[i*12..] = x, y, conic a,b,c, rgb,
+// opacity, depth, ext_x, ext_y (doubles).
+void or_project_f64(const sp_scene *s, const sp_view *v, uint32_t w, uint32_t h, uint8_t *valid, double *proj,
+                    int threads) {
+    threads = resolve_threads(threads);
+    const size_t n = (size_t)s->n;
+    const size_t chunk = 4096;
+    parallel_for((n + chunk - 1) / chunk, threads, [&](size_t c) {
+        for (size_t i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) {
+            P64 p;
+            valid[i] = project64(s, i, *v, w, h, &p) ? 1 : 0;
+            double *o = proj + 12 * i;
+            if (!valid[i]) {
+                for (int k = 0; k < 12; ++k) o[k] = 0;
+                continue;
+            }
+            const double g[12] = {p.x, p.y, p.conic[0], p.conic[1], p.conic[2], p.color[0], p.color[1],
+                                  p.color[2], p.opacity, p.depth, p.ext_x, p.ext_y};
+            for (int k = 0; k < 12; ++k) o[k] = g[k];
+        }
+    });
+}
+
+// FP64 render with the cpu-naive semantics: rgb (h*w*3 doubles, before
+// quantisation, background composited) and the quantised RGB8 image.
+void or_render_f64(const sp_scene *s, const sp_view *v, uint32_t w, uint32_t h, const float bg[3], double *rgb,
+                   uint8_t *out, int threads) {
+    threads = resolve_threads(threads);
+    const size_t n = (size_t)s->n;
+    std::vector<P64> ps;
+    std::vector<uint8_t> ok(n);
+    std::vector<P64> all(n);
+    const size_t chunk = 4096;
+    parallel_for((n + chunk - 1) / chunk, threads, [&](size_t c) {
+        for (size_t i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) ok[i] = project64(s, i, *v, w, h, &all[i]);
+    });
+    std::vector<uint32_t> idx;
+    for (size_t i = 0; i < n; ++i)
+        if (ok[i]) idx.push_back((uint32_t)i);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return all[a].depth < all[b].depth; });
+    const size_t np = (size_t)w * h;
+    std::vector<double> C(3 * np, 0.0), T(np, 1.0);
+    const uint32_t band = 8;
+    const uint32_t nbands = (h + band - 1) / band;
+    parallel_for(nbands, threads, [&](size_t bi) {
+        const uint32_t r0 = (uint32_t)bi * band, r1 = std::min(h, r0 + band);
+        for (uint32_t id : idx) {
+            const P64 &p = all[id];
+            const uint32_t y0 = std::max(p.bbox[1], r0), y1 = std::min(p.bbox[3], r1);
+            for (uint32_t py = y0; py < y1; ++py) {
+                const double dy = py + 0.5 - p.y;
+                for (uint32_t px = p.bbox[0]; px < p.bbox[2]; ++px) {
+                    const size_t q = (size_t)py * w + px;
+                    const double t = T[q];
+                    if (t < (double)T_MIN) continue;
+                    const double dx = px + 0.5 - p.x;
+                    const double pw = -0.5 * (p.conic[0] * dx * dx + p.conic[2] * dy * dy) - p.conic[1] * dx * dy;
+                    if (pw > 0) continue;
+                    const double al = std::min(p.opacity * std::exp(pw), (double)ALPHA_MAX);
+                    if (al < (double)ALPHA_MIN) continue;
+                    for (int c = 0; c < 3; ++c) C[3 * q + c] += p.color[c] * al * t;
+                    T[q] = t * (1.0 - al);
+                }
+            }
+        }
+    });
+    for (size_t q = 0; q < np; ++q)
+        for (int c = 0; c < 3; ++c) {
+            const double val = C[3 * q + c] + T[q] * (double)bg[c];
+            if (rgb) rgb[3 * q + c] = val;
+            if (out) {
+                const double cl = std::min(std::max(val, 0.0), 1.0);
+                out[3 * q + c] = (uint8_t)std::min(255.0, std::floor(cl * 255.0 + 0.5));
+            }
+        }
+}
+
+}  // extern "C"
