@@ -92,6 +92,11 @@ void sp_synth_lattice(uint32_t nx, uint32_t ny, uint32_t nz, float scale, float 
                       float *positions, float *scales, float *quats, float *opacities,
                       float *colors);
 
+/* f32 -> IEEE binary16 / bfloat16, round to nearest even (the conversions the
+ * reduced-precision scene storage uses). */
+void sp_to_f16(const float *in, uint64_t n, uint16_t *out);
+void sp_to_bf16(const float *in, uint64_t n, uint16_t *out);
+
 /* bench::psnr over two RGB8 images of n_pixels pixels (inf when identical, 0 when empty). */
 double sp_psnr(const uint8_t *a, const uint8_t *b, uint64_t n_pixels);
 

@@ -156,6 +156,8 @@ SCENE_SYMBOLS = {
     "sp_rng_normal": (C.c_float, [u64p]),
     "sp_synth_random": (None, [C.POINTER(SpRandomSpec), f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p]),
     "sp_synth_lattice": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float, f32p, f32p, f32p, f32p, f32p]),
+    "sp_to_f16": (None, [f32p, C.c_uint64, u16p]),
+    "sp_to_bf16": (None, [f32p, C.c_uint64, u16p]),
     "sp_psnr": (C.c_double, [u8p, u8p, C.c_uint64]),
 }
 

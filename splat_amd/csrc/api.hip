@@ -1,19 +1,22 @@
-// C ABI of libsplat_hip.so and the per-view orchestration.
+// C ABI of libsplat_hip.so and the orchestration of a batch of views.
 //
 // One handle owns the device copy of a scene plus every per-view buffer, all
 // reused across frames (as CpuTiled reuses its scratch, src/render/cpu.rs:68-79).
-// A view is one stream-ordered sequence of kernels with a single host
-// synchronisation at the end:
-//   project (+compact) -> depth radix sort -> emit pairs -> tile radix sort
+// A batch of views (one view for Renderer::render) is one stream-ordered
+// sequence of launches, each covering every view:
+//   project (+ per-view compaction) -> depth radix sort (one segment per view)
+//   -> emit pairs (all views, one global tile numbering) -> tile radix sort
 //   -> tile ranges -> blend
-// Every data-dependent size (visible count, pair count) stays on the device;
-// grids are sized from capacities and kernels read the live counts. When the
-// pair count exceeds the capacity the frame is re-run with a larger buffer
-// (the reference's gpu backend grows the same way, src/render/gpu.rs:1039-1056).
+// with a single host synchronisation at the end. Every data-dependent size
+// (visible counts, pair count) stays on the device; grids are sized from
+// capacities and kernels read the live counts, and no kernel waits on
+// another block, so a batch can be captured once into a HIP graph and
+// replayed. When the pairs exceed the capacity the batch is re-run with a
+// larger buffer (the reference's gpu backend grows the same way,
+// src/render/gpu.rs:1039-1056).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,12 +24,14 @@
 #include <string>
 #include <vector>
 
+#include "half_conv.h"
 #include "renderer.hpp"
 #include "sp_common.h"
 #include "sp_launch.h"
 #include "splat_hip.h"
 
 using sp::ViewCtx;
+using sp::ViewParams;
 
 namespace {
 
@@ -84,6 +89,17 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// What a captured batch graph depends on besides the contents of the
+// (device-resident) view table.
+struct GraphKey {
+    uint32_t nviews = 0, total_tiles = 0, tile_bits = 0, with_blend = 0;
+    uint64_t generation = 0;
+    bool operator==(const GraphKey &o) const {
+        return nviews == o.nviews && total_tiles == o.total_tiles && tile_bits == o.tile_bits &&
+               with_blend == o.with_blend && generation == o.generation;
+    }
+};
+
 }  // namespace
 
 struct sp_handle {
@@ -93,9 +109,21 @@ struct sp_handle {
     uint32_t n = 0, deg = 0, rest = 0;
     uint64_t stride = 0;
     uint64_t bytes = 0;
+    uint64_t generation = 1;  // bumped whenever a device buffer moves
     float *d_soa = nullptr;
+    uint16_t *d_soa16 = nullptr;
+    uint64_t soa_bytes = 0, soa16_bytes = 0;
+    // Per-view arrays, view_cap views of n entries each.
+    uint32_t view_cap = 0;
     uint4 *d_rec = nullptr;
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
+    uint32_t *d_vstat = nullptr;  // 3 x view_cap: visible, pairs, emission view bases
+    uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
+    ViewParams *d_views = nullptr, *h_views = nullptr;
+    uint32_t *d_hist = nullptr;  // view_cap x 4 passes x 256 digits
+    uint32_t *d_table = nullptr;  // radix per-block digit counts
+    uint64_t table_words = 0;
+    // Pairs of all views.
     uint64_t pair_cap = 0;
     uint32_t *d_pkey[2] = {nullptr, nullptr}, *d_pval[2] = {nullptr, nullptr};
     uint32_t *d_tile_start = nullptr;
@@ -103,12 +131,15 @@ struct sp_handle {
     uint8_t *d_rgb = nullptr;
     uint64_t rgb_cap = 0;
     uint32_t *d_counters = nullptr;
-    unsigned long long *d_status = nullptr;
-    uint64_t status_cap = 0;
-    uint32_t *d_hist = nullptr;
-    uint32_t *h_counters = nullptr;
+    uint32_t *h_counters = nullptr;  // pinned: CNT_SLOTS counters then 2*view_cap view stats
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     sp_frame_info last{};
+    int pair_buf = 0;
+    // Captured batch graph.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    int graph_pair_buf = 0;
 };
 
 namespace {
@@ -121,9 +152,11 @@ int dalloc(sp_handle *h, void **p, uint64_t bytes) {
     const hipError_t e = hipMalloc(p, bytes ? bytes : 16);
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return fail(SP_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
     }
     h->bytes += bytes;
+    h->generation++;
     return SP_OK;
 }
 
@@ -132,30 +165,30 @@ void dfree(sp_handle *h, T *&p, uint64_t bytes) {
     if (p) {
         (void)hipFree(p);
         h->bytes -= bytes;
+        h->generation++;
     }
     p = nullptr;
 }
 
-uint64_t status_words(uint64_t n, uint64_t pair_cap) {
-    const uint64_t scan1 = (n + 255) / 256 + 1;
-    const uint64_t m = std::max(n, pair_cap);
-    const uint64_t radix = ((m + sp::RADIX_TILE - 1) / sp::RADIX_TILE + 1) * sp::RADIX_BINS;
-    return std::max(scan1, radix);
+void drop_graph(sp_handle *h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->gkey = GraphKey{};
 }
 
-int ensure_status(sp_handle *h) {
-    const uint64_t need = status_words(h->n, h->pair_cap);
-    if (need <= h->status_cap) return SP_OK;
-    dfree(h, h->d_status, h->status_cap * 8);
-    h->status_cap = 0;
-    int rc = dalloc(h, (void **)&h->d_status, need * 8);
+// Radix count table large enough for the depth sort of view_cap views and the
+// tile sort of pair_cap pairs.
+int ensure_table(sp_handle *h) {
+    const uint64_t need = std::max(sp::radix_table_words(std::max<uint32_t>(h->view_cap, 1), std::max<uint32_t>(h->n, 1)),
+                                   sp::radix_table_words(1, std::max<uint64_t>(h->pair_cap, 1)));
+    if (need <= h->table_words) return SP_OK;
+    dfree(h, h->d_table, h->table_words * 4);
+    h->table_words = 0;
+    const int rc = dalloc(h, (void **)&h->d_table, need * 4);
     if (rc) return rc;
-    // Zero words read as "not published" in every epoch. The clear is ordered on
-    // the handle's stream (kernels run there) and completed before any launch:
-    // recycled device memory may hold words of another buffer or handle.
-    HIP_TRY(hipMemsetAsync(h->d_status, 0, need * 8, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    h->status_cap = need;
+    h->table_words = need;
     return SP_OK;
 }
 
@@ -179,7 +212,45 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
         }
     }
     h->pair_cap = cap;
-    return ensure_status(h);
+    return ensure_table(h);
+}
+
+// Per-view arrays for `views` views.
+int ensure_views(sp_handle *h, uint32_t views) {
+    if (views <= h->view_cap) return SP_OK;
+    const uint64_t nn = std::max<uint64_t>(h->n, 1);
+    const uint64_t old = h->view_cap;
+    const uint64_t blocks = std::max(sp::project_blocks(h->n), sp::emit_blocks(h->n));
+    dfree(h, h->d_rec, old * nn * 48);
+    for (int b = 0; b < 2; ++b) {
+        dfree(h, h->d_dkey[b], old * nn * 4);
+        dfree(h, h->d_dval[b], old * nn * 4);
+    }
+    dfree(h, h->d_vstat, old * 12);
+    dfree(h, h->d_bcnt, old * blocks * 4);
+    dfree(h, h->d_views, old * sizeof(ViewParams));
+    dfree(h, h->d_hist, old * 4 * sp::RADIX_BINS * 4);
+    if (h->h_views) (void)hipHostFree(h->h_views);
+    if (h->h_counters) (void)hipHostFree(h->h_counters);
+    h->h_views = nullptr;
+    h->h_counters = nullptr;
+    h->view_cap = 0;
+    const uint64_t v = views;
+    int rc = dalloc(h, (void **)&h->d_rec, v * nn * 48);
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
+    }
+    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 12);
+    if (!rc) rc = dalloc(h, (void **)&h->d_bcnt, v * blocks * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_views, v * sizeof(ViewParams));
+    if (!rc) rc = dalloc(h, (void **)&h->d_hist, v * 4 * sp::RADIX_BINS * 4);
+    if (rc) return rc;
+    if (hipHostMalloc((void **)&h->h_views, v * sizeof(ViewParams), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_counters, (sp::CNT_SLOTS + 2 * v) * 4, hipHostMallocDefault) != hipSuccess)
+        return fail(SP_E_NOMEM, "hipHostMalloc failed");
+    h->view_cap = views;
+    return ensure_table(h);
 }
 
 int ensure_tiles(sp_handle *h, uint64_t tiles) {
@@ -205,11 +276,12 @@ int ensure_rgb(sp_handle *h, uint64_t bytes) {
 sp::DevScene dev_scene(const sp_handle *h) {
     sp::DevScene s;
     s.soa = h->d_soa;
+    s.soa16 = h->d_soa16;
     s.stride = h->stride;
     s.n = h->n;
     s.deg = h->deg;
     s.rest = h->rest;
-    s.pad = 0;
+    s.storage = (uint32_t)h->opt.storage;
     return s;
 }
 
@@ -220,41 +292,105 @@ int check_frame(uint32_t w, uint32_t hgt) {
     return SP_OK;
 }
 
-// Run one view. d_out == nullptr stops after the tile ranges (inspection).
-// On return *pair_buf names the ping-pong buffer that holds the sorted pairs.
-int run_view(sp_handle *h, const sp_view &view, uint32_t w, uint32_t hgt, const float bg[3], uint8_t *d_out,
-             sp_stats *st, int *pair_buf) {
-    const double t0 = now_ms();
-    const ViewCtx ctx = make_ctx(view, w, hgt, bg);
-    const uint32_t num_tiles = ctx.tiles_x * ctx.tiles_y;
-    int rc = ensure_tiles(h, num_tiles);
-    if (rc) return rc;
+int setup(sp_handle *h) {
+    HIP_TRY(hipSetDevice(h->device));
+    return SP_OK;
+}
+
+// Enqueue the whole pipeline for the first `nviews` entries of h_views. The
+// sorted pairs end in ping-pong buffer *pair_buf.
+int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, bool blend, bool events,
+            int *pair_buf) {
     const sp::DevScene sc = dev_scene(h);
-    const int tile_bits = (int)bits_for(num_tiles - 1);
+    hipStream_t s = h->stream;
+    uint32_t *vis = h->d_vstat;
+    uint32_t *view_pairs = h->d_vstat + h->view_cap;
+    uint32_t *vtot = h->d_vstat + 2 * h->view_cap;
+    HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
+    if (events) HIP_TRY(hipEventRecord(h->ev[0], s));
+    // The uncompacted depth keys live in the depth sort's second buffer until the sort needs it.
+    HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_dkey[1], h->d_bcnt, vis, h->d_dkey[0],
+                               h->d_dval[0], nullptr, nullptr, nullptr));
+    int di = 0;
+    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_hist, h->d_table, &di));
+    HIP_TRY(sp::launch_emit(s, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, h->d_bcnt, vtot,
+                            h->d_pkey[0], h->d_pval[0], h->pair_cap, h->d_counters));
+    if (events) HIP_TRY(hipEventRecord(h->ev[1], s));
+    int pi = 0;
+    HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
+                                  tile_bits, h->d_hist, h->d_table, &pi));
+    HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters, h->pair_cap, total_tiles, h->d_tile_start));
+    HIP_TRY(sp::launch_view_pairs(s, h->d_views, nviews, total_tiles, h->d_tile_start, view_pairs));
+    if (events) HIP_TRY(hipEventRecord(h->ev[2], s));
+    if (blend)
+        HIP_TRY(sp::launch_blend(s, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi], h->d_rec,
+                                 h->d_counters));
+    if (events) HIP_TRY(hipEventRecord(h->ev[3], s));
+    HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, sp::CNT_SLOTS * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS, vis, nviews * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS + nviews, view_pairs, nviews * 4, hipMemcpyDeviceToHost, s));
+    *pair_buf = pi;
+    return SP_OK;
+}
+
+// Render `nviews` views (outs[v] device RGB8 buffers; blend == false stops
+// after the tile ranges). use_graph: capture the batch once and replay it.
+int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_t *wh, const float bg[3],
+              uint8_t *const *outs, bool blend, bool use_graph, sp_stats *stats) {
+    const double t0 = now_ms();
+    if (nviews == 0) return SP_OK;
+    if (nviews > sp::MAX_VIEWS) return fail(SP_E_INVALID, "too many views in one batch");
+    if ((uint64_t)nviews * std::max<uint32_t>(h->n, 1) > 0xFFFFFFFFull)
+        return fail(SP_E_INVALID, "views x gaussians exceeds 2^32 records");
+    int rc = ensure_views(h, nviews);
+    if (rc) return rc;
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        ViewParams &p = h->h_views[v];
+        std::memset(&p, 0, sizeof p);
+        p.ctx = make_ctx(views[v], wh[2 * v], wh[2 * v + 1], bg);
+        p.tile_base = (uint32_t)total;
+        p.out = outs ? outs[v] : nullptr;
+        total += (uint64_t)p.ctx.tiles_x * p.ctx.tiles_y;
+    }
+    if (total >= 0xFFFFFFFFull) return fail(SP_E_INVALID, "too many tiles in one batch");
+    const uint32_t total_tiles = (uint32_t)total;
+    rc = ensure_tiles(h, total_tiles);
+    if (rc) return rc;
+    const int tile_bits = (int)bits_for(total_tiles ? total_tiles - 1 : 0);
+    const bool events = h->opt.timing && !use_graph;
     for (int attempt = 0;; ++attempt) {
-        // Built per attempt: growing the pair buffers reallocates the status array.
-        const sp::Lookback lb{h->d_status, h->d_counters};
-        if (h->opt.timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        HIP_TRY(sp::launch_project(h->stream, sc, ctx, h->d_rec, h->d_dkey[0], h->d_dval[0], lb, nullptr, nullptr,
-                                   nullptr));
-        int di = 0;
-        HIP_TRY(sp::launch_radix_sort(h->stream, h->d_dkey, h->d_dval, h->d_counters + sp::CNT_VISIBLE, h->n, 32, lb,
-                                      h->d_hist, &di));
-        HIP_TRY(sp::launch_emit(h->stream, h->d_dval[di], h->n, h->d_rec, ctx, h->d_pkey[0], h->d_pval[0],
-                                h->pair_cap, lb));
-        if (h->opt.timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
-        int pi = 0;
-        HIP_TRY(sp::launch_radix_sort(h->stream, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, h->pair_cap,
-                                      tile_bits, lb, h->d_hist, &pi));
-        HIP_TRY(sp::launch_ranges(h->stream, h->d_pkey[pi], h->d_counters, h->pair_cap, num_tiles, h->d_tile_start));
-        if (h->opt.timing) HIP_TRY(hipEventRecord(h->ev[2], h->stream));
-        if (d_out)
-            HIP_TRY(sp::launch_blend(h->stream, ctx, h->d_tile_start, h->d_pval[pi], h->d_rec, d_out,
-                                     h->d_counters));
-        if (h->opt.timing) HIP_TRY(hipEventRecord(h->ev[3], h->stream));
-        HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, sp::CNT_SLOTS * 4, hipMemcpyDeviceToHost, h->stream));
+        int pb = 0;
+        if (use_graph) {
+            GraphKey key;
+            key.nviews = nviews;
+            key.total_tiles = total_tiles;
+            key.tile_bits = (uint32_t)tile_bits;
+            key.with_blend = blend ? 1u : 0u;
+            key.generation = h->generation;
+            if (!h->gexec || !(key == h->gkey)) {
+                drop_graph(h);
+                HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue(h, nviews, total_tiles, tile_bits, blend, false, &h->graph_pair_buf);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ce != hipSuccess) return fail(SP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+                h->graph = g;
+                HIP_TRY(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+                h->gkey = key;
+            }
+            HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+            pb = h->graph_pair_buf;
+        } else {
+            rc = enqueue(h, nviews, total_tiles, tile_bits, blend, events, &pb);
+            if (rc) return rc;
+        }
         HIP_TRY(hipStreamSynchronize(h->stream));
-        *pair_buf = pi;
+        h->pair_buf = pb;
         if (h->h_counters[sp::CNT_OVERFLOW] && attempt < 3) {
             const uint64_t p = h->h_counters[sp::CNT_PAIRS];
             rc = ensure_pairs(h, p + p / 4 + 1024);
@@ -264,33 +400,95 @@ int run_view(sp_handle *h, const sp_view &view, uint32_t w, uint32_t hgt, const 
         if (h->h_counters[sp::CNT_OVERFLOW]) return fail(SP_E_CAPACITY, "pair buffer overflow persisted");
         break;
     }
-    if (st) {
-        st->visible = h->h_counters[sp::CNT_VISIBLE];
-        st->overlaps = h->h_counters[sp::CNT_PAIRS];
-        if (h->opt.timing) {
-            float a = 0, b = 0, c = 0;
-            HIP_TRY(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-            HIP_TRY(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-            HIP_TRY(hipEventElapsedTime(&c, h->ev[2], h->ev[3]));
-            st->project_ms = a;
-            st->sort_ms = b;
-            st->raster_ms = c;
-        }
-        st->total_ms = now_ms() - t0;
+    float tp = 0, ts = 0, tr = 0;
+    if (events) {
+        HIP_TRY(hipEventElapsedTime(&tp, h->ev[0], h->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&ts, h->ev[1], h->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&tr, h->ev[2], h->ev[3]));
     }
-    h->last.width = w;
-    h->last.height = hgt;
-    h->last.tiles = num_tiles;
-    h->last.visible = h->h_counters[sp::CNT_VISIBLE];
+    const double total_ms = now_ms() - t0;
+    const uint32_t *vis = h->h_counters + sp::CNT_SLOTS;
+    const uint32_t *vpairs = vis + nviews;
+    uint64_t vis_sum = 0;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        vis_sum += vis[v];
+        if (stats) {
+            sp_stats &st = stats[v];
+            st.visible = vis[v];
+            st.overlaps = vpairs[v];
+            // Stage times cover the whole launch sequence (all views of the batch).
+            st.project_ms = tp;
+            st.sort_ms = ts;
+            st.raster_ms = tr;
+            st.total_ms = total_ms;
+        }
+    }
+    h->last.width = wh[0];
+    h->last.height = wh[1];
+    h->last.tiles = total_tiles;
+    h->last.visible = vis_sum;
     h->last.pairs = h->h_counters[sp::CNT_PAIRS];
-    h->last.pairs_staged = d_out ? h->h_counters[sp::CNT_STAGED] : 0;
+    h->last.pairs_staged = blend ? h->h_counters[sp::CNT_STAGED] : 0;
     h->last.pair_capacity = h->pair_cap;
     h->last.device_bytes = h->bytes;
     return SP_OK;
 }
 
-int setup(sp_handle *h) {
-    HIP_TRY(hipSetDevice(h->device));
+int render_one(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, const float bg[3],
+               uint8_t *d_out, sp_stats *stats) {
+    const uint32_t wh[2] = {width, height};
+    uint8_t *outs[1] = {d_out};
+    return run_views(h, 1, view, wh, bg, outs, d_out != nullptr, false, stats);
+}
+
+// Upload the scene in the handle's storage format.
+int upload_scene(sp_handle *h, const sp_scene *scene) {
+    const uint32_t n = h->n, rest = h->rest;
+    const uint64_t S = h->stride;
+    if (h->opt.storage == SP_STORE_F32) {
+        const uint64_t ncols = (uint64_t)sp::A_SH + 3ull * rest;
+        std::vector<float> soa(ncols * S, 0.0f);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) soa[(sp::A_POS + c) * S + i] = scene->positions[3ull * i + c];
+            for (int c = 0; c < 6; ++c) soa[(sp::A_COV + c) * S + i] = scene->cov3d[6ull * i + c];
+            const float o = scene->opacities[i];
+            soa[sp::A_OPAC * S + i] = o;
+            soa[sp::A_K * S + i] = k_of(o);
+            for (int c = 0; c < 3; ++c) soa[(sp::A_COL + c) * S + i] = scene->colors[3ull * i + c];
+            for (uint32_t k = 0; k < 3 * rest; ++k)
+                soa[(sp::A_SH + k) * S + i] = scene->sh_rest[(uint64_t)3 * rest * i + k];
+        }
+        h->soa_bytes = soa.size() * 4;
+        int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(h->d_soa, soa.data(), h->soa_bytes, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return SP_OK;
+    }
+    // 16-bit storage: f32 position, opacity, k; 16-bit scale, quaternion, colour, SH.
+    uint16_t (*cvt)(float) = h->opt.storage == SP_STORE_F16 ? sp::f32_to_f16 : sp::f32_to_bf16;
+    std::vector<float> soa(5 * S, 0.0f);
+    const uint64_t hcols = (uint64_t)sp::H_SH + 3ull * rest;
+    std::vector<uint16_t> soa16(hcols * S, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) soa[(sp::L_POS + c) * S + i] = scene->positions[3ull * i + c];
+        const float o = scene->opacities[i];
+        soa[sp::L_OPAC * S + i] = o;
+        soa[sp::L_K * S + i] = k_of(o);
+        for (int c = 0; c < 3; ++c) soa16[(sp::H_SCALE + c) * S + i] = cvt(scene->scales[3ull * i + c]);
+        for (int c = 0; c < 4; ++c) soa16[(sp::H_QUAT + c) * S + i] = cvt(scene->rotations[4ull * i + c]);
+        for (int c = 0; c < 3; ++c) soa16[(sp::H_COL + c) * S + i] = cvt(scene->colors[3ull * i + c]);
+        for (uint32_t k = 0; k < 3 * rest; ++k)
+            soa16[(sp::H_SH + k) * S + i] = cvt(scene->sh_rest[(uint64_t)3 * rest * i + k]);
+    }
+    h->soa_bytes = soa.size() * 4;
+    h->soa16_bytes = soa16.size() * 2;
+    int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
+    if (!rc) rc = dalloc(h, (void **)&h->d_soa16, h->soa16_bytes);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_soa, soa.data(), h->soa_bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_soa16, soa16.data(), h->soa16_bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return SP_OK;
 }
 
@@ -313,7 +511,9 @@ void sp_destroy(sp_handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
     (void)hipFree(h->d_soa);
+    (void)hipFree(h->d_soa16);
     (void)hipFree(h->d_rec);
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(h->d_dkey[b]);
@@ -321,12 +521,16 @@ void sp_destroy(sp_handle *h) {
         (void)hipFree(h->d_pkey[b]);
         (void)hipFree(h->d_pval[b]);
     }
+    (void)hipFree(h->d_vstat);
+    (void)hipFree(h->d_bcnt);
+    (void)hipFree(h->d_views);
+    (void)hipFree(h->d_table);
     (void)hipFree(h->d_tile_start);
     (void)hipFree(h->d_rgb);
     (void)hipFree(h->d_counters);
-    (void)hipFree(h->d_status);
     (void)hipFree(h->d_hist);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
+    if (h->h_views) (void)hipHostFree(h->h_views);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -344,12 +548,16 @@ int sp_create(const sp_scene *scene, const sp_options *opt_in, sp_handle **out) 
         sp_default_options(&opt);
     if (scene->sh_degree > 3) return fail(SP_E_INVALID, "sh_degree must be 0..3");
     if (scene->n > 0x7FFFFFFFull) return fail(SP_E_INVALID, "more than 2^31 gaussians");
+    if (opt.storage != SP_STORE_F32 && opt.storage != SP_STORE_F16 && opt.storage != SP_STORE_BF16)
+        return fail(SP_E_INVALID, "unknown storage precision");
     const uint32_t n = (uint32_t)scene->n;
     const uint32_t rest = (scene->sh_degree + 1) * (scene->sh_degree + 1) - 1;
-    if (n && (!scene->positions || !scene->cov3d || !scene->opacities || !scene->colors))
+    if (n && (!scene->positions || !scene->opacities || !scene->colors))
         return fail(SP_E_INVALID, "scene arrays missing");
+    if (n && opt.storage == SP_STORE_F32 && !scene->cov3d) return fail(SP_E_INVALID, "cov3d missing");
+    if (n && opt.storage != SP_STORE_F32 && (!scene->scales || !scene->rotations))
+        return fail(SP_E_INVALID, "16-bit storage needs scales and rotations");
     if (n && rest && !scene->sh_rest) return fail(SP_E_INVALID, "sh_rest missing for sh_degree > 0");
-    if (opt.storage != SP_STORE_F32) return fail(SP_E_UNSUPPORTED, "storage precision not supported yet");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SP_E_DEVICE, "no HIP device");
     if (opt.device < 0 || opt.device >= count) return fail(SP_E_DEVICE, "device ordinal out of range");
@@ -370,53 +578,18 @@ int sp_create(const sp_scene *scene, const sp_options *opt_in, sp_handle **out) 
         return bail(fail(SP_E_DEVICE, "hipStreamCreate failed"));
     for (auto &e : h->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(SP_E_DEVICE, "hipEventCreate failed"));
-
     // Structure-of-arrays layout (one 64-aligned array per field).
-    h->stride = ((uint64_t)n + 63) / 64 * 64;
-    if (h->stride == 0) h->stride = 64;
-    const uint64_t ncols = (uint64_t)sp::A_SH + 3ull * rest;
-    {
-        std::vector<float> soa(ncols * h->stride, 0.0f);
-        const uint64_t S = h->stride;
-        for (uint32_t i = 0; i < n; ++i) {
-            for (int c = 0; c < 3; ++c) soa[(sp::A_POS + c) * S + i] = scene->positions[3ull * i + c];
-            for (int c = 0; c < 6; ++c) soa[(sp::A_COV + c) * S + i] = scene->cov3d[6ull * i + c];
-            const float o = scene->opacities[i];
-            soa[sp::A_OPAC * S + i] = o;
-            soa[sp::A_K * S + i] = k_of(o);
-            for (int c = 0; c < 3; ++c) soa[(sp::A_COL + c) * S + i] = scene->colors[3ull * i + c];
-            for (uint32_t k = 0; k < 3 * rest; ++k)
-                soa[(sp::A_SH + k) * S + i] = scene->sh_rest[(uint64_t)3 * rest * i + k];
-        }
-        rc = dalloc(h, (void **)&h->d_soa, soa.size() * 4);
-        if (rc) return bail(rc);
-        if (hipMemcpyAsync(h->d_soa, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipStreamSynchronize(h->stream) != hipSuccess)
-            return bail(fail(SP_E_DEVICE, "scene upload failed"));
-    }
-    const uint64_t nn = std::max<uint64_t>(n, 1);
-    rc = dalloc(h, (void **)&h->d_rec, nn * 48);
-    for (int b = 0; b < 2 && !rc; ++b) {
-        rc = dalloc(h, (void **)&h->d_dkey[b], nn * 4);
-        if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], nn * 4);
-    }
+    h->stride = std::max<uint64_t>(((uint64_t)n + 63) / 64 * 64, 64);
+    rc = upload_scene(h, scene);
     if (!rc) rc = dalloc(h, (void **)&h->d_counters, sp::CNT_SLOTS * 4);
-    if (!rc) rc = dalloc(h, (void **)&h->d_hist, 4 * sp::RADIX_BINS * 4);
     if (rc) return bail(rc);
-    // Each handle starts its lookback epochs at a distinct base, so status words
-    // left in recycled memory by another handle never carry a live epoch tag.
-    static std::atomic<uint32_t> g_handles{0};
-    uint32_t init[sp::CNT_SLOTS] = {0};
-    init[sp::CNT_EPOCH] = ((g_handles.fetch_add(1) * 0x9E3779B1u) >> 2 | 1u) & 0x3FFFFFFFu;
-    if (hipMemcpyAsync(h->d_counters, init, sizeof init, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemsetAsync(h->d_hist, 0, 4 * sp::RADIX_BINS * 4, h->stream) != hipSuccess ||
+    const uint32_t zero[sp::CNT_SLOTS] = {0};
+    if (hipMemcpyAsync(h->d_counters, zero, sizeof zero, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(SP_E_DEVICE, "counter init failed"));
-    if (hipHostMalloc((void **)&h->h_counters, sp::CNT_SLOTS * 4, hipHostMallocDefault) != hipSuccess)
-        return bail(fail(SP_E_DEVICE, "hipHostMalloc failed"));
+    rc = ensure_views(h, 1);
     const uint64_t cap0 = opt.initial_pairs ? opt.initial_pairs : std::max<uint64_t>(16ull * n, 1ull << 20);
-    rc = ensure_pairs(h, cap0);
-    if (!rc) rc = ensure_status(h);
+    if (!rc) rc = ensure_pairs(h, cap0);
     if (rc) return bail(rc);
     *out = h;
     return SP_OK;
@@ -432,8 +605,7 @@ int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     if (!rgb_out_device) return fail(SP_E_INVALID, "sp_render_device: NULL output");
     rc = setup(h);
     if (rc) return rc;
-    int pb = 0;
-    return run_view(h, *view, width, height, background, rgb_out_device, stats, &pb);
+    return render_one(h, view, width, height, background, rgb_out_device, stats);
 }
 
 int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, const float background[3],
@@ -450,8 +622,7 @@ int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height
     rc = ensure_rgb(h, bytes);
     if (rc) return rc;
     const double t0 = now_ms();
-    int pb = 0;
-    rc = run_view(h, *view, width, height, background, h->d_rgb, stats, &pb);
+    rc = render_one(h, view, width, height, background, h->d_rgb, stats);
     if (rc) return rc;
     HIP_TRY(hipMemcpy(rgb_out, h->d_rgb, bytes, hipMemcpyDeviceToHost));
     if (stats) stats->total_ms = now_ms() - t0;
@@ -468,18 +639,21 @@ int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     const uint64_t n = h->n;
     if (n == 0) return SP_OK;
     const float bg[3] = {0.f, 0.f, 0.f};
-    const ViewCtx ctx = make_ctx(*view, width, height, bg);
+    ViewParams *p = h->h_views;
+    std::memset(p, 0, sizeof *p);
+    p->ctx = make_ctx(*view, width, height, bg);
     float *d_dbg = nullptr;
     uint16_t *d_bb = nullptr;
     uint8_t *d_valid = nullptr;
     HIP_TRY(hipMalloc(&d_dbg, n * 48));
     HIP_TRY(hipMalloc(&d_bb, n * 8));
     HIP_TRY(hipMalloc(&d_valid, n));
-    HIP_TRY(hipMemsetAsync(d_dbg, 0, n * 48, h->stream));
-    HIP_TRY(hipMemsetAsync(d_bb, 0, n * 8, h->stream));
-    const sp::Lookback lb{h->d_status, h->d_counters};
-    hipError_t e = sp::launch_project(h->stream, dev_scene(h), ctx, h->d_rec, h->d_dkey[0], h->d_dval[0], lb, d_dbg,
-                                      d_bb, d_valid);
+    hipError_t e = hipMemsetAsync(d_dbg, 0, n * 48, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_bb, 0, n * 8, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->d_views, h->h_views, sizeof(ViewParams), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_dkey[1], h->d_bcnt,
+                               h->d_vstat, h->d_dkey[0], h->d_dval[0], d_dbg, d_bb, d_valid);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(proj, d_dbg, n * 48, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(bbox, d_bb, n * 8, hipMemcpyDeviceToHost);
@@ -502,8 +676,7 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     if (rc) return rc;
     const float bg[3] = {0.f, 0.f, 0.f};
     sp_stats st;
-    int pb = 0;
-    rc = run_view(h, *view, width, height, bg, nullptr, &st, &pb);
+    rc = render_one(h, view, width, height, bg, nullptr, &st);
     if (rc) return rc;
     const uint64_t np = st.overlaps;
     *n_pairs = np;
@@ -514,8 +687,8 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     if (!keys || !vals) return fail(SP_E_INVALID, "sp_debug_pairs: NULL keys/vals");
     std::vector<uint32_t> tk(np);
     std::vector<uint4> rec((uint64_t)h->n * 3);
-    HIP_TRY(hipMemcpy(tk.data(), h->d_pkey[pb], np * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(vals, h->d_pval[pb], np * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tk.data(), h->d_pkey[h->pair_buf], np * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vals, h->d_pval[h->pair_buf], np * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(rec.data(), h->d_rec, rec.size() * 16, hipMemcpyDeviceToHost));
     for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | rec[3ull * vals[p] + 2].w;
     return SP_OK;
@@ -531,14 +704,50 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
                     const float background[3], uint8_t *const *outs, int32_t outs_on_device, sp_stats *stats) {
     if (!h || (!views && n_views) || (!wh && n_views) || !background || (!outs && n_views))
         return fail(SP_E_INVALID, "sp_render_batch: NULL argument");
+    if (stats)
+        for (uint32_t v = 0; v < n_views; ++v) std::memset(&stats[v], 0, sizeof(sp_stats));
+    if (n_views == 0) return SP_OK;
+    int rc = setup(h);
+    if (rc) return rc;
+    // Zero-sized views render nothing (as Renderer::render); the others form the batch.
+    std::vector<sp_view> bv;
+    std::vector<uint32_t> bwh;
+    std::vector<uint32_t> idx;
+    uint64_t host_bytes = 0;
     for (uint32_t v = 0; v < n_views; ++v) {
-        const int rc = outs_on_device
-                           ? sp_render_device(h, &views[v], wh[2 * v], wh[2 * v + 1], background, outs[v],
-                                              stats ? &stats[v] : nullptr)
-                           : sp_render(h, &views[v], wh[2 * v], wh[2 * v + 1], background, outs[v],
-                                       stats ? &stats[v] : nullptr);
+        rc = check_frame(wh[2 * v], wh[2 * v + 1]);
         if (rc) return rc;
+        if (wh[2 * v] == 0 || wh[2 * v + 1] == 0) continue;
+        if (!outs[v]) return fail(SP_E_INVALID, "sp_render_batch: NULL output");
+        bv.push_back(views[v]);
+        bwh.push_back(wh[2 * v]);
+        bwh.push_back(wh[2 * v + 1]);
+        idx.push_back(v);
+        host_bytes += 3ull * wh[2 * v] * wh[2 * v + 1];
     }
+    const uint32_t nb = (uint32_t)idx.size();
+    if (nb == 0) return SP_OK;
+    std::vector<uint8_t *> douts(nb);
+    if (outs_on_device) {
+        for (uint32_t k = 0; k < nb; ++k) douts[k] = outs[idx[k]];
+    } else {
+        rc = ensure_rgb(h, host_bytes);
+        if (rc) return rc;
+        uint64_t o = 0;
+        for (uint32_t k = 0; k < nb; ++k) {
+            douts[k] = h->d_rgb + o;
+            o += 3ull * bwh[2 * k] * bwh[2 * k + 1];
+        }
+    }
+    std::vector<sp_stats> st(nb);
+    rc = run_views(h, nb, bv.data(), bwh.data(), background, douts.data(), true, true, st.data());
+    if (rc) return rc;
+    if (!outs_on_device) {
+        for (uint32_t k = 0; k < nb; ++k)
+            HIP_TRY(hipMemcpy(outs[idx[k]], douts[k], 3ull * bwh[2 * k] * bwh[2 * k + 1], hipMemcpyDeviceToHost));
+    }
+    if (stats)
+        for (uint32_t k = 0; k < nb; ++k) stats[idx[k]] = st[k];
     return SP_OK;
 }
 

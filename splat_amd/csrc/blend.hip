@@ -1,6 +1,7 @@
 // (e) Per-tile front-to-back alpha blending.
 //
-// One 256-thread workgroup per 16x16 tile, one pixel per thread; wave k of the
+// One 256-thread workgroup per 16x16 tile of any view of the batch (the views'
+// tiles are numbered consecutively), one pixel per thread; wave k of the
 // group owns tile rows 4k..4k+3. The tile's depth-ordered splat list is
 // staged through LDS 256 records at a time (each thread gathers one 48-byte
 // record), then every wave walks the batch: a record whose pixel box misses
@@ -21,16 +22,27 @@ namespace {
 
 constexpr int XCDS = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs
 
-__global__ __launch_bounds__(BLEND_THREADS) void k_blend(ViewCtx v, const uint32_t *__restrict__ tile_start,
+__global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
+                                                         uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
-                                                         const uint4 *__restrict__ rec,
-                                                         uint8_t *__restrict__ rgb, uint32_t per_xcd,
+                                                         const uint4 *__restrict__ rec, uint32_t per_xcd,
                                                          uint32_t *__restrict__ counters) {
     __shared__ uint4 s_rec[BLEND_THREADS][3];
-    // Consecutive tiles (rows of the image) on the same XCD share its L2.
-    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
-    const uint32_t num_tiles = v.tiles_x * v.tiles_y;
-    if (t >= num_tiles) return;
+    // Consecutive tiles (rows of an image) on the same XCD share its L2.
+    const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (gt >= total_tiles) return;
+    // The view owning global tile gt: last view whose tile_base <= gt (bases ascend).
+    uint32_t lo = 0, hi = nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].tile_base <= gt)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const ViewCtx &v = views[lo].ctx;
+    uint8_t *__restrict__ rgb = views[lo].out;
+    const uint32_t t = gt - views[lo].tile_base;
     const uint32_t tx = t % v.tiles_x, ty = t / v.tiles_x;
     const uint32_t ox = tx * TILE, oy = ty * TILE;
     const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
@@ -42,7 +54,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(ViewCtx v, const uint32
     const float lane = (float)(lx & 3u);
     const float pyc = ((float)oy + (float)ly) + 0.5f;
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    const uint32_t start = tile_start[t], end = tile_start[t + 1];
+    const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
     uint32_t b = start;
     for (; b < end; b += BLEND_THREADS) {
         const uint32_t idx = b + threadIdx.x;
@@ -100,13 +112,12 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(ViewCtx v, const uint32
 
 }  // namespace
 
-hipError_t launch_blend(hipStream_t s, const ViewCtx &v, const uint32_t *tile_start, const uint32_t *sval,
-                        const uint4 *rec, uint8_t *rgb, uint32_t *counters) {
-    const uint32_t num_tiles = v.tiles_x * v.tiles_y;
-    if (num_tiles == 0) return hipSuccess;
-    const uint32_t per_xcd = (num_tiles + XCDS - 1) / XCDS;
-    hipLaunchKernelGGL(k_blend, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, v, tile_start, sval, rec, rgb,
-                       per_xcd, counters);
+hipError_t launch_blend(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters) {
+    if (total_tiles == 0) return hipSuccess;
+    const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
+    hipLaunchKernelGGL(k_blend, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, views, nviews, total_tiles,
+                       tile_start, sval, rec, per_xcd, counters);
     return hipGetLastError();
 }
 

@@ -1,8 +1,16 @@
-// (a) EWA projection + SH colour (degrees 0-3) + compaction of visible splats.
+// (a) EWA projection + SH colour (degrees 0-3) + per-view compaction of the
+// visible splats, for every view of a batch in one launch.
 //
-// One thread per splat, structure-of-arrays reads (every field of splat i is
-// a 4-byte load at array_base + i, so a wave reads 256 contiguous bytes per
-// field). The arithmetic is render::project (src/render/mod.rs:201-272) and
+// One thread per splat. The thread reads its splat once — structure-of-arrays
+// loads (each field of splat i is a 4-byte load at array_base + i, so a wave
+// reads 256 contiguous bytes per field) kept in registers, SH included — and
+// projects it into every view, PROJECT_VCHUNK views per round. Each view's
+// visible splats are then compacted in scene-index order: per-block counts
+// (wave ballots), one scan workgroup per view over its block counts, and a
+// compaction pass that places each visible splat at its block offset plus
+// its in-block ballot rank.
+//
+// The arithmetic is render::project (src/render/mod.rs:201-272) and
 // sh::eval_rest (src/sh.rs:39-64) in the reference's operation order, with
 // contraction disabled, so every field is bit-identical to the CPU backend.
 #include "sp_device.h"
@@ -17,10 +25,14 @@ const __device__ float SH_C3[7] = {-0.5900436f, 2.8906114f, -0.4570458f, 0.37317
                                    -0.4570458f, 1.4453057f, -0.5900436f};
 
 template <int DEG>
-__device__ __forceinline__ void sh_color(const float *A, uint64_t S, uint32_t i, float x, float y, float z,
-                                         float c[3]) {
-    // Coefficient k, channel ch of splat i.
-    auto r = [&](int k, int ch) { return A[(uint64_t)(A_SH + 3 * k + ch) * S + i]; };
+constexpr int rest_of() {
+    return (DEG + 1) * (DEG + 1) - 1;
+}
+
+// colour += eval_rest(deg, rest, dir); sh holds coefficient k channel ch at 3k+ch.
+template <int DEG>
+__device__ __forceinline__ void sh_color(const float *sh, float x, float y, float z, float c[3]) {
+    auto r = [&](int k, int ch) { return sh[3 * k + ch]; };
     float e[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) e[ch] = SH_C1 * (((-y) * r(0, ch) + z * r(1, ch)) - x * r(2, ch));
@@ -63,158 +75,310 @@ __device__ __forceinline__ void sh_color(const float *A, uint64_t S, uint32_t i,
     for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + e[ch];
 }
 
+// A splat's parameters, loaded once.
 template <int DEG>
-__global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, ViewCtx v, uint4 *__restrict__ rec,
-                                                             uint32_t *__restrict__ dkey,
-                                                             uint32_t *__restrict__ dval, Lookback lb,
+struct Splat {
+    float opacity, k, p[3], cv[6], col[3];
+    float sh[3 * rest_of<DEG>() + 1];
+};
+
+// 16-bit storage decode.
+template <int STORE>
+__device__ __forceinline__ float h2f(uint16_t h) {
+    if (STORE == SP_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// cov3d from scale and a (re-normalised) rotation, f32, in the operation order
+// of scene::covariance (src/scene.rs:170-178) and Scene::finalize's
+// normalisation (src/scene.rs:72-79).
+__device__ __forceinline__ void covariance_f32(const float sc[3], float q[4], float cv[6]) {
+    const float len = sqrtf((q[0] * q[0] + q[2] * q[2]) + (q[1] * q[1] + q[3] * q[3]));
+    if (__builtin_isfinite(len) && len > 1e-12f) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q[c] = q[c] / len;
+    } else {
+        q[0] = q[1] = q[2] = 0.0f;
+        q[3] = 1.0f;
+    }
+    const float x2 = q[0] + q[0], y2 = q[1] + q[1], z2 = q[2] + q[2];
+    const float xx = q[0] * x2, xy = q[0] * y2, xz = q[0] * z2;
+    const float yy = q[1] * y2, yz = q[1] * z2, zz = q[2] * z2;
+    const float wx = q[3] * x2, wy = q[3] * y2, wz = q[3] * z2;
+    const float r[3][3] = {{1.0f - (yy + zz), xy + wz, xz - wy},
+                           {xy - wz, 1.0f - (xx + zz), yz + wx},
+                           {xz + wy, yz - wx, 1.0f - (xx + yy)}};  // r[col][row]
+    float m[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            m[j][c] = (r[0][c] * (j == 0 ? sc[0] : 0.0f) + r[1][c] * (j == 1 ? sc[1] : 0.0f)) +
+                      r[2][c] * (j == 2 ? sc[2] : 0.0f);
+    float s[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[j][c] = (m[0][c] * m[0][j] + m[1][c] * m[1][j]) + m[2][c] * m[2][j];
+    cv[0] = s[0][0];
+    cv[1] = s[1][0];
+    cv[2] = s[2][0];
+    cv[3] = s[1][1];
+    cv[4] = s[2][1];
+    cv[5] = s[2][2];
+}
+
+template <int DEG, int STORE>
+__device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat<DEG> &s) {
+    const float *A = sc.soa;
+    const uint64_t S = sc.stride;
+    if (STORE == SP_STORE_F32) {
+        s.opacity = A[(uint64_t)A_OPAC * S + i];
+        s.k = A[(uint64_t)A_K * S + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.p[c] = A[(uint64_t)(A_POS + c) * S + i];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s.cv[c] = A[(uint64_t)(A_COV + c) * S + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.col[c] = A[(uint64_t)(A_COL + c) * S + i];
+#pragma unroll
+        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = A[(uint64_t)(A_SH + c) * S + i];
+    } else {
+        const uint16_t *B = sc.soa16;
+        s.opacity = A[(uint64_t)L_OPAC * S + i];
+        s.k = A[(uint64_t)L_K * S + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.p[c] = A[(uint64_t)(L_POS + c) * S + i];
+        float scl[3], q[4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) scl[c] = h2f<STORE>(B[(uint64_t)(H_SCALE + c) * S + i]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q[c] = h2f<STORE>(B[(uint64_t)(H_QUAT + c) * S + i]);
+        covariance_f32(scl, q, s.cv);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.col[c] = h2f<STORE>(B[(uint64_t)(H_COL + c) * S + i]);
+#pragma unroll
+        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = h2f<STORE>(B[(uint64_t)(H_SH + c) * S + i]);
+    }
+}
+
+// render::project of one splat into one view. Writes the 48-byte record and
+// returns true when the splat survives culling.
+template <int DEG, bool PREC>
+__device__ __forceinline__ bool project_view(const Splat<DEG> &s, const ViewCtx &v, uint4 *o, float *depth,
+                                             float *dbg, uint16_t *dbg_bbox) {
+    const float d0 = s.p[0] - v.eye[0], d1 = s.p[1] - v.eye[1], d2 = s.p[2] - v.eye[2];
+    // t = rot * (p - eye), glam column order.
+    const float tx = (v.rot[0][0] * d0 + v.rot[0][1] * d1) + v.rot[0][2] * d2;
+    const float ty = (v.rot[1][0] * d0 + v.rot[1][1] * d1) + v.rot[1][2] * d2;
+    const float tz = (v.rot[2][0] * d0 + v.rot[2][1] * d1) + v.rot[2][2] * d2;
+    bool ok = !(s.opacity < ALPHA_MIN) && tz > v.near_z;
+    const float inv_z = 1.0f / tz;
+    const float x = v.focal * tx * inv_z + v.cx;
+    const float y = v.focal * ty * inv_z + v.cy;
+    float jx = tx * inv_z, jy = ty * inv_z;  // f32::clamp
+    if (jx < -v.lim_x) jx = -v.lim_x;
+    if (jx > v.lim_x) jx = v.lim_x;
+    if (jy < -v.lim_y) jy = -v.lim_y;
+    if (jy > v.lim_y) jy = v.lim_y;
+    const float f = v.focal * inv_z;
+    const float j0z = -f * jx, j1z = -f * jy;
+    // Rows of T = J * R: (rot^T * j)[c] = right[c]*j.x + down[c]*j.y + forward[c]*j.z.
+    float t0[3], t1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        t0[c] = (v.rot[0][c] * f + v.rot[1][c] * 0.0f) + v.rot[2][c] * j0z;
+        t1[c] = (v.rot[0][c] * 0.0f + v.rot[1][c] * f) + v.rot[2][c] * j1z;
+    }
+    const float *cv = s.cv;
+    const float s00 = (cv[0] * t0[0] + cv[1] * t0[1]) + cv[2] * t0[2];
+    const float s01 = (cv[1] * t0[0] + cv[3] * t0[1]) + cv[4] * t0[2];
+    const float s02 = (cv[2] * t0[0] + cv[4] * t0[1]) + cv[5] * t0[2];
+    const float s10 = (cv[0] * t1[0] + cv[1] * t1[1]) + cv[2] * t1[2];
+    const float s11 = (cv[1] * t1[0] + cv[3] * t1[1]) + cv[4] * t1[2];
+    const float s12 = (cv[2] * t1[0] + cv[4] * t1[1]) + cv[5] * t1[2];
+    const float a = ((t0[0] * s00 + t0[1] * s01) + t0[2] * s02) + DILATION;
+    const float b = (t1[0] * s00 + t1[1] * s01) + t1[2] * s02;
+    const float cc = ((t1[0] * s10 + t1[1] * s11) + t1[2] * s12) + DILATION;
+    const float det = a * cc - b * b;
+    float inv_det;
+    if (PREC && !(det > NEAR_SINGULAR * (a * cc))) {
+        // Near-singular: the f32 determinant is mostly cancellation error.
+        const double d64 = (double)a * (double)cc - (double)b * (double)b;
+        ok = ok && d64 > 0.0;
+        inv_det = (float)(1.0 / d64);
+    } else {
+        ok = ok && det > 0.0f;
+        inv_det = 1.0f / det;
+    }
+    const float ext_x = s.k * sqrtf(a);
+    const float ext_y = s.k * sqrtf(cc);
+    ok = ok && !(x + ext_x <= 0.0f || x - ext_x >= v.width || y + ext_y <= 0.0f || y - ext_y >= v.height);
+    ok = ok && __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(ext_x) &&
+         __builtin_isfinite(ext_y);
+    if (!ok) return false;
+    float col[3] = {s.col[0], s.col[1], s.col[2]};
+    if (DEG > 0) {
+        // normalize_or_zero(p - eye) (glam): d * (1/|d|) when that is finite and > 0.
+        const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+        const float rcp = 1.0f / len;
+        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+        if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+            ux = d0 * rcp;
+            uy = d1 * rcp;
+            uz = d2 * rcp;
+        }
+        sh_color<DEG>(s.sh, ux, uy, uz, col);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
+    const float ca = cc * inv_det, cb = -b * inv_det, cc2 = a * inv_det;
+    // cpu::pixel_bbox (src/render/cpu.rs:96-102).
+    const uint32_t bx0 = sat_u32(fmaxf(floorf(x - ext_x), 0.0f));
+    const uint32_t by0 = sat_u32(fmaxf(floorf(y - ext_y), 0.0f));
+    const uint32_t bx1 = min(sat_u32(fmaxf(ceilf(x + ext_x), 0.0f)), v.w);
+    const uint32_t by1 = min(sat_u32(fmaxf(ceilf(y + ext_y), 0.0f)), v.h);
+    const uint32_t lo = min(bx0, bx1) | (min(by0, by1) << 16);
+    const uint32_t hi = bx1 | (by1 << 16);
+    o[0] = make_uint4(__float_as_uint(x), __float_as_uint(y), __float_as_uint(-0.5f * ca), __float_as_uint(-cb));
+    o[1] = make_uint4(__float_as_uint(-0.5f * cc2), __float_as_uint(s.opacity), __float_as_uint(col[0]),
+                      __float_as_uint(col[1]));
+    o[2] = make_uint4(__float_as_uint(col[2]), lo, hi, __float_as_uint(tz));
+    *depth = tz;
+    if (dbg) {
+        const float g[12] = {x, y, ca, cb, cc2, col[0], col[1], col[2], s.opacity, tz, ext_x, ext_y};
+#pragma unroll
+        for (int q = 0; q < 12; ++q) dbg[q] = g[q];
+        dbg_bbox[0] = (uint16_t)min(bx0, bx1);
+        dbg_bbox[1] = (uint16_t)min(by0, by1);
+        dbg_bbox[2] = (uint16_t)bx1;
+        dbg_bbox[3] = (uint16_t)by1;
+    }
+    return true;
+}
+
+// Projects every splat into every view. Writes the records of visible
+// splats, full[v*n + i] = depth bits (or FULL_CULLED) and, per view, the
+// block's visible count bcnt[v * gridDim.x + block].
+template <int DEG, int STORE>
+__global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, const ViewParams *__restrict__ views,
+                                                             uint32_t nviews, uint4 *__restrict__ rec,
+                                                             uint32_t *__restrict__ full,
+                                                             uint32_t *__restrict__ bcnt,
                                                              float *__restrict__ dbg,
                                                              uint16_t *__restrict__ dbg_bbox,
                                                              uint8_t *__restrict__ dbg_valid) {
-    const Ticket tk = acquire_block(lb.counters);
-    const uint32_t i = tk.bid * PROJECT_THREADS + threadIdx.x;
-    const float *A = sc.soa;
-    const uint64_t S = sc.stride;
-    bool vis = false;
-    float depth = 0.0f;
-    if (i < sc.n) {
-        const float opacity = A[(uint64_t)A_OPAC * S + i];
-        const float px = A[(uint64_t)(A_POS + 0) * S + i];
-        const float py = A[(uint64_t)(A_POS + 1) * S + i];
-        const float pz = A[(uint64_t)(A_POS + 2) * S + i];
-        float cv[6];
+    constexpr int NW = PROJECT_THREADS / 64;
+    __shared__ uint32_t s_wcnt[PROJECT_VCHUNK][NW];
+    const uint32_t i = blockIdx.x * PROJECT_THREADS + threadIdx.x;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t N = sc.n;
+    Splat<DEG> s;
+    const bool have = i < sc.n;
+    if (have) load_splat<DEG, STORE>(sc, i, s);
+    for (uint32_t v0 = 0; v0 < nviews; v0 += PROJECT_VCHUNK) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cv[k] = A[(uint64_t)(A_COV + k) * S + i];
-        const float kk = A[(uint64_t)A_K * S + i];
-        const float d0 = px - v.eye[0], d1 = py - v.eye[1], d2 = pz - v.eye[2];
-        // t = rot * (p - eye), glam column order.
-        const float tx = (v.rot[0][0] * d0 + v.rot[0][1] * d1) + v.rot[0][2] * d2;
-        const float ty = (v.rot[1][0] * d0 + v.rot[1][1] * d1) + v.rot[1][2] * d2;
-        const float tz = (v.rot[2][0] * d0 + v.rot[2][1] * d1) + v.rot[2][2] * d2;
-        bool ok = !(opacity < ALPHA_MIN) && tz > v.near_z;
-        const float inv_z = 1.0f / tz;
-        const float x = v.focal * tx * inv_z + v.cx;
-        const float y = v.focal * ty * inv_z + v.cy;
-        float jx = tx * inv_z, jy = ty * inv_z;  // f32::clamp
-        if (jx < -v.lim_x) jx = -v.lim_x;
-        if (jx > v.lim_x) jx = v.lim_x;
-        if (jy < -v.lim_y) jy = -v.lim_y;
-        if (jy > v.lim_y) jy = v.lim_y;
-        const float f = v.focal * inv_z;
-        const float j0z = -f * jx, j1z = -f * jy;
-        // Rows of T = J * R: (rot^T * j)[c] = right[c]*j.x + down[c]*j.y + forward[c]*j.z.
-        float t0[3], t1[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            t0[c] = (v.rot[0][c] * f + v.rot[1][c] * 0.0f) + v.rot[2][c] * j0z;
-            t1[c] = (v.rot[0][c] * 0.0f + v.rot[1][c] * f) + v.rot[2][c] * j1z;
-        }
-        const float s00 = (cv[0] * t0[0] + cv[1] * t0[1]) + cv[2] * t0[2];
-        const float s01 = (cv[1] * t0[0] + cv[3] * t0[1]) + cv[4] * t0[2];
-        const float s02 = (cv[2] * t0[0] + cv[4] * t0[1]) + cv[5] * t0[2];
-        const float s10 = (cv[0] * t1[0] + cv[1] * t1[1]) + cv[2] * t1[2];
-        const float s11 = (cv[1] * t1[0] + cv[3] * t1[1]) + cv[4] * t1[2];
-        const float s12 = (cv[2] * t1[0] + cv[4] * t1[1]) + cv[5] * t1[2];
-        const float a = ((t0[0] * s00 + t0[1] * s01) + t0[2] * s02) + DILATION;
-        const float b = (t1[0] * s00 + t1[1] * s01) + t1[2] * s02;
-        const float cc = ((t1[0] * s10 + t1[1] * s11) + t1[2] * s12) + DILATION;
-        const float det = a * cc - b * b;
-        ok = ok && det > 0.0f;
-        const float inv_det = 1.0f / det;
-        const float ext_x = kk * sqrtf(a);
-        const float ext_y = kk * sqrtf(cc);
-        ok = ok && !(x + ext_x <= 0.0f || x - ext_x >= v.width || y + ext_y <= 0.0f || y - ext_y >= v.height);
-        ok = ok && __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(ext_x) &&
-             __builtin_isfinite(ext_y);
-        if (ok) {
-            float col[3] = {A[(uint64_t)(A_COL + 0) * S + i], A[(uint64_t)(A_COL + 1) * S + i],
-                            A[(uint64_t)(A_COL + 2) * S + i]};
-            if (DEG > 0) {
-                // normalize_or_zero(p - eye) (glam): d * (1/|d|) when that is finite and > 0.
-                const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-                const float rcp = 1.0f / len;
-                float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-                if (__builtin_isfinite(rcp) && rcp > 0.0f) {
-                    ux = d0 * rcp;
-                    uy = d1 * rcp;
-                    uz = d2 * rcp;
-                }
-                sh_color<DEG>(A, S, i, ux, uy, uz, col);
+        for (int c = 0; c < PROJECT_VCHUNK; ++c) {
+            const uint32_t v = v0 + c;
+            bool vis = false;
+            if (have && v < nviews) {
+                float *g = (dbg && v == 0) ? dbg + 12 * (uint64_t)i : nullptr;
+                uint16_t *gb = (dbg && v == 0) ? dbg_bbox + 4 * (uint64_t)i : nullptr;
+                float depth = 0.0f;
+                vis = project_view<DEG, STORE != SP_STORE_F32>(s, views[v].ctx, rec + 3 * ((uint64_t)v * N + i),
+                                                                &depth, g, gb);
+                full[(uint64_t)v * N + i] = vis ? __float_as_uint(depth) : FULL_CULLED;
+                if (dbg_valid && v == 0) dbg_valid[i] = vis ? 1 : 0;
             }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
-            const float ca = cc * inv_det, cb = -b * inv_det, cc2 = a * inv_det;
-            // cpu::pixel_bbox (src/render/cpu.rs:96-102).
-            const uint32_t bx0 = sat_u32(fmaxf(floorf(x - ext_x), 0.0f));
-            const uint32_t by0 = sat_u32(fmaxf(floorf(y - ext_y), 0.0f));
-            const uint32_t bx1 = min(sat_u32(fmaxf(ceilf(x + ext_x), 0.0f)), v.w);
-            const uint32_t by1 = min(sat_u32(fmaxf(ceilf(y + ext_y), 0.0f)), v.h);
-            const uint32_t lo = min(bx0, bx1) | (min(by0, by1) << 16);
-            const uint32_t hi = bx1 | (by1 << 16);
-            uint4 *o = rec + 3 * (uint64_t)i;
-            o[0] = make_uint4(__float_as_uint(x), __float_as_uint(y), __float_as_uint(-0.5f * ca),
-                              __float_as_uint(-cb));
-            o[1] = make_uint4(__float_as_uint(-0.5f * cc2), __float_as_uint(opacity), __float_as_uint(col[0]),
-                              __float_as_uint(col[1]));
-            o[2] = make_uint4(__float_as_uint(col[2]), lo, hi, __float_as_uint(tz));
-            vis = true;
-            depth = tz;
-            if (dbg) {
-                float *g = dbg + 12 * (uint64_t)i;
-                g[0] = x;
-                g[1] = y;
-                g[2] = ca;
-                g[3] = cb;
-                g[4] = cc2;
-                g[5] = col[0];
-                g[6] = col[1];
-                g[7] = col[2];
-                g[8] = opacity;
-                g[9] = tz;
-                g[10] = ext_x;
-                g[11] = ext_y;
-                uint16_t *bb = dbg_bbox + 4 * (uint64_t)i;
-                bb[0] = (uint16_t)min(bx0, bx1);
-                bb[1] = (uint16_t)min(by0, by1);
-                bb[2] = (uint16_t)bx1;
-                bb[3] = (uint16_t)by1;
-            }
+            const unsigned long long m = __ballot(vis);
+            if (lane == 0) s_wcnt[c][w] = (uint32_t)__popcll(m);
         }
-        if (dbg_valid) dbg_valid[i] = vis ? 1 : 0;
+        __syncthreads();
+        if (threadIdx.x < PROJECT_VCHUNK && v0 + threadIdx.x < nviews) {
+            uint32_t t = 0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) t += s_wcnt[threadIdx.x][ww];
+            bcnt[(uint64_t)(v0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+        }
+        __syncthreads();
     }
-    uint32_t total;
-    const uint32_t local = block_exclusive_scan<PROJECT_THREADS>(vis ? 1u : 0u, &total);
-    const uint32_t excl = lookback_scan1(lb.status, tk.bid, tk.epoch, total);
-    if (vis && excl + local < sc.n) {  // always in range for a consistent scan; guards the buffer
-        dkey[excl + local] = __float_as_uint(depth);
-        dval[excl + local] = i;
+}
+
+// Per-view compaction of the visible splats in scene-index order: block b of
+// view v reads full[v*n + b*256 ..] and writes (depth bits, scene index) at
+// the view's scanned block offset plus the in-block ballot rank.
+__global__ __launch_bounds__(PROJECT_THREADS) void k_compact(const uint32_t *__restrict__ full, uint32_t n,
+                                                             uint32_t nblocks, const uint32_t *__restrict__ boff,
+                                                             uint32_t *__restrict__ dkey,
+                                                             uint32_t *__restrict__ dval) {
+    constexpr int NW = PROJECT_THREADS / 64;
+    __shared__ uint32_t s_w[NW];
+    const uint32_t v = blockIdx.x / nblocks, b = blockIdx.x % nblocks;
+    const uint32_t i = b * PROJECT_THREADS + threadIdx.x;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t vb = (uint64_t)v * n;
+    const uint32_t key = i < n ? full[vb + i] : FULL_CULLED;
+    const bool vis = key != FULL_CULLED;
+    const unsigned long long m = __ballot(vis);
+    const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (lane == 0) s_w[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t wo = 0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) wo += (uint32_t)ww < w ? s_w[ww] : 0u;
+    if (vis) {
+        const uint32_t pos = boff[(uint64_t)v * nblocks + b] + wo + r;
+        if (pos < n) {  // always in range for consistent counts; guards the buffer
+            dkey[vb + pos] = key;
+            dval[vb + pos] = i;
+        }
     }
-    if (tk.bid == gridDim.x - 1 && threadIdx.x == 0) lb.counters[CNT_VISIBLE] = excl + total;
-    finish_block(lb.counters, tk.epoch);
+}
+
+template <int DEG, int STORE>
+void launch_deg(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewParams *views, uint32_t nviews,
+                uint4 *rec, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+    hipLaunchKernelGGL((k_project<DEG, STORE>), dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, views, nviews, rec,
+                       full, bcnt, dbg, dbg_bbox, dbg_valid);
+}
+
+template <int STORE>
+void launch_store(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewParams *views, uint32_t nviews,
+                  uint4 *rec, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+    switch (sc.deg) {
+    case 0: launch_deg<0, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    case 1: launch_deg<1, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    case 2: launch_deg<2, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    default: launch_deg<3, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    }
 }
 
 }  // namespace
 
-hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewCtx &v, uint4 *rec, uint32_t *dkey,
-                          uint32_t *dval, Lookback lb, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
-    const uint32_t grid = sc.n ? (sc.n + PROJECT_THREADS - 1) / PROJECT_THREADS : 1;
-    switch (sc.deg) {
-    case 0:
-        hipLaunchKernelGGL(k_project<0>, dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, v, rec, dkey, dval, lb, dbg,
-                           dbg_bbox, dbg_valid);
+uint32_t project_blocks(uint32_t n) { return n ? (n + PROJECT_THREADS - 1) / PROJECT_THREADS : 1; }
+
+hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
+                          uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey, uint32_t *dval,
+                          float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+    const uint32_t grid = project_blocks(sc.n);
+    switch (sc.storage) {
+    case SP_STORE_F16:
+        launch_store<SP_STORE_F16>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
-    case 1:
-        hipLaunchKernelGGL(k_project<1>, dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, v, rec, dkey, dval, lb, dbg,
-                           dbg_bbox, dbg_valid);
-        break;
-    case 2:
-        hipLaunchKernelGGL(k_project<2>, dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, v, rec, dkey, dval, lb, dbg,
-                           dbg_bbox, dbg_valid);
+    case SP_STORE_BF16:
+        launch_store<SP_STORE_BF16>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     default:
-        hipLaunchKernelGGL(k_project<3>, dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, v, rec, dkey, dval, lb, dbg,
-                           dbg_bbox, dbg_valid);
+        launch_store<SP_STORE_F32>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // Per view: exclusive scan of the block counts; row totals are the visible counts.
+    e = launch_scan_rows(s, bcnt, grid, nviews, nullptr, vis_count);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact, dim3(grid * nviews), dim3(PROJECT_THREADS), 0, s, full, sc.n, grid, bcnt, dkey,
+                       dval);
     return hipGetLastError();
 }
 

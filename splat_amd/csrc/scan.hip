@@ -1,0 +1,104 @@
+// Device-wide exclusive scans in reduce-then-scan form (no cross-block
+// waiting): producers write one count per block into a row; these kernels
+// scan the rows (one workgroup per row, all rows in parallel); consumers
+// re-read their block's offset. Used by the projection compaction, the pair
+// emission and the radix sort.
+#include <algorithm>
+
+#include "sp_device.h"
+#include "sp_launch.h"
+
+namespace sp {
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;  // entries per thread per chunk
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+
+// Block-wide exclusive scan of SCAN_ITEMS consecutive entries per thread of
+// `row` starting at `off` (entries beyond `len` read as 0); returns the chunk
+// total. Adds `carry` to every written prefix.
+__device__ __forceinline__ uint32_t scan_chunk(uint32_t *row, uint64_t off, uint64_t len, uint32_t carry,
+                                               bool write) {
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+    const uint64_t b = off + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (b + k < len) ? row[b + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan<SCAN_THREADS>(sum, &total) + carry;
+    if (write) {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (b + k < len) row[b + k] = pre;
+            pre += v[k];
+        }
+    }
+    return total;
+}
+
+// One workgroup per row of `row_len` entries: in-place exclusive scan, plus
+// row_base[r] (when given) added to every entry; row_total[r] (when given)
+// receives the row's sum.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict__ data, uint64_t row_len,
+                                                            const uint32_t *__restrict__ row_base,
+                                                            uint32_t *__restrict__ row_total) {
+    uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
+    uint32_t carry = row_base ? row_base[blockIdx.x] : 0u;
+    const uint32_t base = carry;
+    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) carry += scan_chunk(row, off, row_len, carry, true);
+    if (row_total && threadIdx.x == 0) row_total[blockIdx.x] = carry - base;
+}
+
+// Radix digit rows: row r = (segment g, digit d) of `row_len` per-block counts;
+// the row's base is the exclusive prefix of the segment's digit histogram
+// hist[g * hist_seg_stride + 0..255] below d.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
+                                                                  const uint32_t *__restrict__ hist,
+                                                                  uint32_t hist_seg_stride) {
+    __shared__ uint32_t s_base;
+    const uint32_t g = blockIdx.x / RADIX_BINS, d = blockIdx.x % RADIX_BINS;
+    uint32_t tot;
+    const uint32_t pre = block_exclusive_scan<SCAN_THREADS>(hist[(uint64_t)g * hist_seg_stride + threadIdx.x], &tot);
+    if (threadIdx.x == d) s_base = pre;
+    __syncthreads();
+    uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
+    uint32_t carry = s_base;
+    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) carry += scan_chunk(row, off, row_len, carry, true);
+}
+
+__global__ __launch_bounds__(256) void k_fill(uint32_t *__restrict__ p, uint64_t n, uint32_t value) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+}
+
+}  // namespace
+
+static_assert(RADIX_BINS == SCAN_THREADS, "digit-row bases use one thread per digit");
+
+hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
+                            const uint32_t *row_base, uint32_t *row_total) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_rows, dim3(rows), dim3(SCAN_THREADS), 0, s, data, row_len, row_base, row_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
+                                  const uint32_t *hist, uint32_t hist_seg_stride) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len, hist,
+                       hist_seg_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, p, n, value);
+    return hipGetLastError();
+}
+
+}  // namespace sp

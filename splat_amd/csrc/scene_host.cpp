@@ -12,6 +12,7 @@
 #include <limits>
 #include <vector>
 
+#include "half_conv.h"
 #include "splat_scene.h"
 
 namespace {
@@ -295,6 +296,14 @@ void sp_synth_lattice(uint32_t nx, uint32_t ny, uint32_t nz, float scale, float 
                 quats[4 * i + 3] = 1.f;
                 opacities[i] = opacity;
             }
+}
+
+void sp_to_f16(const float *in, uint64_t n, uint16_t *out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = sp::f32_to_f16(in[i]);
+}
+
+void sp_to_bf16(const float *in, uint64_t n, uint16_t *out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = sp::f32_to_bf16(in[i]);
 }
 
 double sp_psnr(const uint8_t *a, const uint8_t *b, uint64_t n_pixels) {

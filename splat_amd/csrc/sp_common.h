@@ -3,6 +3,8 @@
 
 #include <stdint.h>
 
+#include "splat_hip.h"
+
 namespace sp {
 
 // Reference constants (src/render/mod.rs:23-33, src/render/cpu.rs:34).
@@ -14,27 +16,24 @@ constexpr uint32_t TILE = 16;
 
 // Workgroup shapes.
 constexpr int PROJECT_THREADS = 256;  // one splat per thread
+constexpr int PROJECT_VCHUNK = 8;     // views projected per pass over a splat held in registers
 constexpr int EMIT_THREADS = 256;     // one depth-ordered splat per thread
 constexpr int BLEND_THREADS = 256;    // one pixel of a 16x16 tile per thread
 constexpr int RADIX_THREADS = 256;    // one digit per thread in the lookback
 constexpr int RADIX_ITEMS = 16;       // keys per thread per onesweep tile
 constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ITEMS;  // 4096 keys
 constexpr int RADIX_BINS = 256;       // 8-bit digits
+constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
 // Device counter slots (uint32 each) in the per-handle counter block.
 enum Counter : int {
-    CNT_VISIBLE = 0,   // splats that survived culling (V)
-    CNT_PAIRS = 1,     // (splat, tile) pairs (P), exact even when over capacity
-    CNT_OVERFLOW = 2,  // 1 when P exceeded the pair capacity
+    CNT_PAIRS = 1,     // (splat, tile) pairs over all views, exact even when over capacity
+    CNT_OVERFLOW = 2,  // 1 when the pairs exceeded the pair capacity
     CNT_STAGED = 3,    // pair records the blend gathered (early termination skips the rest)
-    CNT_TICKET = 4,    // dynamic block-id ticket of the running lookback kernel
-    CNT_DONE = 5,      // finished blocks of the running lookback kernel
-    CNT_EPOCH = 6,     // launch epoch tag for lookback status words
     CNT_SLOTS = 16
 };
 
-// ProjectCtx (src/render/mod.rs:155-197) plus frame/tile geometry, passed by
-// value to every per-view kernel.
+// ProjectCtx (src/render/mod.rs:155-197) plus frame/tile geometry.
 struct ViewCtx {
     float rot[3][3];  // rows: right, down, forward (world -> camera)
     float eye[3];
@@ -44,34 +43,52 @@ struct ViewCtx {
     uint32_t pad;
 };
 
+// One view of a batch, in device memory: its context, the first global tile id
+// of its tiles (views' tiles are numbered consecutively) and its RGB8 output.
+struct ViewParams {
+    ViewCtx ctx;
+    uint32_t tile_base;
+    uint32_t pad0;
+    uint8_t *out;
+};
+
 // Device-resident scene in structure-of-arrays form: array `a` of n floats
 // starts at soa + a * stride. Arrays: 0-2 position x,y,z; 3-8 cov3d
 // xx,xy,xz,yy,yz,zz; 9 opacity; 10 k = sqrt(2 ln(o/ALPHA_MIN)); 11-13 base
 // colour; 14.. SH rest coefficient c channel ch at 14 + 3c + ch.
+//
+// Reduced-precision storage (FP16 / BF16) keeps the f32 arrays 0-2 position,
+// 3 opacity, 4 k, and stores the covariance factors and colour as 16-bit
+// arrays in soa16: 0-2 scale, 3-6 quaternion x,y,z,w, 7-9 base colour,
+// 10.. SH rest (coefficient c channel ch at 10 + 3c + ch). The covariance is
+// rebuilt in f32 on the device.
 struct DevScene {
     const float *soa;
+    const uint16_t *soa16;
     uint64_t stride;
     uint32_t n;
     uint32_t deg;
     uint32_t rest;
-    uint32_t pad;
+    uint32_t storage;  // SP_STORE_*
 };
 constexpr int A_POS = 0, A_COV = 3, A_OPAC = 9, A_K = 10, A_COL = 11, A_SH = 14;
+constexpr int L_POS = 0, L_OPAC = 3, L_K = 4;                            // f32 arrays, 16-bit storage
+constexpr int H_SCALE = 0, H_QUAT = 3, H_COL = 7, H_SH = 10;             // 16-bit arrays
+// det <= NEAR_SINGULAR * a * c marks a projected covariance whose f32
+// determinant lost more than ~10 of its 24 bits to cancellation; the 16-bit
+// storage paths recompute such determinants in FP64.
+constexpr float NEAR_SINGULAR = 1e-3f;
 
-// Projected splat record (48 bytes, indexed by scene index):
+// Projected splat record (48 bytes); view v's record of scene index i is at
+// v * n + i. Layout:
 //   r0 = x, y, -0.5*conic_a, -conic_b
 //   r1 = -0.5*conic_c, opacity, red, green
 //   r2 = blue, bits(x0 | y0 << 16), bits(x1 | y1 << 16), depth
 // The pre-scaled conic terms are the exact factors raster_tile multiplies by
 // (src/render/cpu.rs:662, :671-672), so precomputing them changes no bits.
-struct Rec {
-    float r[12];
-};
 
-// Lookback-scan state shared by the single-pass scans and the onesweep sort.
-struct Lookback {
-    unsigned long long *status;  // epoch-tagged status words
-    uint32_t *counters;          // CNT_* slots (ticket, done, epoch)
-};
+// Depth-key marker of a culled (view, splat) in the uncompacted key array;
+// visible depths are positive finite floats, whose bits never reach it.
+constexpr uint32_t FULL_CULLED = 0xFFFFFFFFu;
 
 }  // namespace sp

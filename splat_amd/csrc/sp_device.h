@@ -1,5 +1,5 @@
-// Device-side building blocks: lookback scans, block scans, the exact exp
-// polynomial of the reference blend, saturating conversions.
+// Device-side building blocks: wave and block scans, the exact exp polynomial
+// of the reference blend, saturating conversions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,64 +7,6 @@
 #include "sp_common.h"
 
 namespace sp {
-
-// ---- epoch-tagged lookback status words -----------------------------------
-// word = epoch(30) << 34 | flag(2) << 32 | value(32). flag 0 = not published,
-// 1 = block aggregate, 2 = inclusive prefix. A word from another launch (other
-// epoch) reads as not published, so the status array never needs clearing.
-constexpr unsigned long long FLAG_AGG = 1ull, FLAG_INC = 2ull;
-
-__device__ __forceinline__ unsigned long long pack_status(uint32_t ep, unsigned long long flag, uint32_t v) {
-    return ((unsigned long long)(ep & 0x3FFFFFFFu) << 34) | (flag << 32) | (unsigned long long)v;
-}
-__device__ __forceinline__ bool status_ready(unsigned long long s, uint32_t ep) {
-    return (uint32_t)(s >> 34) == (ep & 0x3FFFFFFFu) && ((s >> 32) & 3ull) != 0;
-}
-__device__ __forceinline__ unsigned long long ld_status(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-struct Ticket {
-    uint32_t bid;
-    uint32_t epoch;
-};
-
-// Dynamic block id: blocks take ids in the order they start, so a block only
-// ever waits on blocks that are already resident (forward progress without
-// assumptions about dispatch order).
-__device__ __forceinline__ Ticket acquire_block(uint32_t *counters) {
-    __shared__ uint32_t s_bid, s_ep;
-    if (threadIdx.x == 0) {
-        s_bid = __hip_atomic_fetch_add(&counters[CNT_TICKET], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_ep = __hip_atomic_load(&counters[CNT_EPOCH], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    return Ticket{s_bid, s_ep};
-}
-
-// Called by every block at its end. The last block to finish resets the
-// ticket/done counters and advances the epoch, so the next launch (also a
-// replay of a captured graph) starts clean. Returns true in that block.
-__device__ __forceinline__ bool finish_block(uint32_t *counters, uint32_t ep) {
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t d =
-            __hip_atomic_fetch_add(&counters[CNT_DONE], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = d == gridDim.x - 1;
-        if (s_last) {
-            __hip_atomic_store(&counters[CNT_TICKET], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&counters[CNT_DONE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&counters[CNT_EPOCH], (ep + 1u) & 0x3FFFFFFFu, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    __syncthreads();
-    return s_last != 0;
-}
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -86,7 +28,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
 }
 
 // Exclusive scan of one u32 per thread over a block of NT threads (NT a
-// multiple of 64). All threads must call it.
+// multiple of 64). All threads must call it; *total receives the block sum.
 template <int NT>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
     constexpr int NW = NT / 64;
@@ -105,51 +47,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     __syncthreads();
     *total = tot;
     return pre + inc - v;
-}
-
-// Decoupled lookback for one value per block, done by wave 0 (64
-// predecessors per step). Returns the sum of `aggregate` over blocks with a
-// smaller id, in every thread.
-__device__ __forceinline__ uint32_t lookback_scan1(unsigned long long *status, uint32_t bid, uint32_t ep,
-                                                   uint32_t aggregate) {
-    __shared__ uint32_t s_excl;
-    if (threadIdx.x < 64) {
-        const uint32_t lane = lane_id();
-        if (bid == 0) {
-            if (lane == 0) {
-                st_status(&status[0], pack_status(ep, FLAG_INC, aggregate));
-                s_excl = 0;
-            }
-        } else {
-            if (lane == 0) st_status(&status[bid], pack_status(ep, FLAG_AGG, aggregate));
-            uint32_t excl = 0;
-            int64_t j = (int64_t)bid - 1;
-            for (;;) {
-                const int64_t idx = j - (int64_t)lane;
-                const unsigned long long s = idx >= 0 ? ld_status(&status[idx]) : pack_status(ep, FLAG_INC, 0);
-                const bool ready = status_ready(s, ep);
-                const bool inc = ready && ((s >> 32) & 3ull) == FLAG_INC;
-                const unsigned long long incm = __ballot(inc);
-                const unsigned long long notready = __ballot(!ready);
-                const int fi = incm ? __ffsll((long long)incm) - 1 : 64;
-                const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1ull);
-                if (notready & need) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                const uint32_t v = (int)lane <= fi ? (uint32_t)s : 0u;
-                excl += wave_sum(v);
-                if (fi < 64) break;
-                j -= 64;
-            }
-            if (lane == 0) {
-                st_status(&status[bid], pack_status(ep, FLAG_INC, excl + aggregate));
-                s_excl = excl;
-            }
-        }
-    }
-    __syncthreads();
-    return s_excl;
 }
 
 // Rust `as u32` on f32 (saturating, NaN -> 0).

@@ -1,4 +1,6 @@
 // Host-side launchers of the rasterizer kernels (one translation unit each).
+// Every per-view array is laid out view-major with a stride of n (the scene
+// size): view v's entry for scene index i is at v * n + i.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,34 +9,60 @@
 
 namespace sp {
 
-// (a) Projection + SH colour + visible-splat compaction. Writes records for
-// visible splats, depth keys/values (scene index) in scene-index order, and
-// counters[CNT_VISIBLE]. dbg / dbg_bbox / dbg_valid (optional, n entries) get
-// the full per-splat projection for inspection.
-hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewCtx &v, uint4 *rec, uint32_t *dkey,
-                          uint32_t *dval, Lookback lb, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
+// ---- scan.hip: reduce-then-scan building blocks ----
+// In-place exclusive scan of `rows` rows of row_len u32 (one workgroup per
+// row); row_base[r] (optional) is added; row_total[r] (optional) gets the sum.
+hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
+                            const uint32_t *row_base, uint32_t *row_total);
+// Radix tables: rows (segment g, digit d), based at the exclusive prefix of
+// hist[g * hist_seg_stride + 0..255] below d.
+hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
+                                  const uint32_t *hist, uint32_t hist_seg_stride);
+// p[0..n) = value (a kernel, so it may be captured in a graph).
+hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 
-// (c) Stable LSD radix sort of u32 keys with u32 values over bits [0, bits),
-// 8-bit digits, onesweep passes with decoupled lookback. The element count is
-// read from *n_dev; n_cap bounds it and sizes the grid. Ping-pongs between
-// keys[0]/keys[1]; returns in *out_index which buffer holds the result.
-// `hist` holds ceil(bits/8)*256 zeroed u32 and is zeroed again on return.
-hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *n_dev,
-                             uint64_t n_cap, int bits, Lookback lb, uint32_t *hist, int *out_index);
+// ---- project.hip: (a) projection + SH colour + per-view compaction ----
+uint32_t project_blocks(uint32_t n);
+// Projects every splat into each of `nviews` views: records rec[3*(v*n+i)..]
+// for visible splats; full[v*n + i] depth bits or FULL_CULLED; bcnt (nviews
+// rows of project_blocks(n) block counts, scanned in place); vis_count[v];
+// compacted depth keys dkey[v*n + j] with scene indices dval[v*n + j] for
+// j < vis_count[v], in scene-index order. dbg / dbg_bbox / dbg_valid
+// (optional, n entries) get view 0's full per-splat projection.
+hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
+                          uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey, uint32_t *dval,
+                          float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
 
-// (b) Tile-pair emission in depth order: for the depth-sorted splats (values
-// dval, count counters[CNT_VISIBLE]) write one (tile id, scene index) pair per
-// overlapped 16x16 tile, compacted by a single-pass scan. Sets
-// counters[CNT_PAIRS] (exact) and counters[CNT_OVERFLOW] when above cap.
-hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint64_t n_cap, const uint4 *rec, const ViewCtx &v,
-                       uint32_t *pkey, uint32_t *pval, uint64_t pair_cap, Lookback lb);
+// ---- radix.hip: (c) stable LSD radix sort ----
+// Words of the per-block digit-count table for nseg segments of up to n_cap keys.
+uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap);
+// Sorts u32 keys with u32 values over bits [0, bits), 8-bit digits, over
+// `nseg` independent segments: segment g holds min(seg_counts[g], n_cap) keys
+// at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
+// between keys[0]/keys[1]; *out_index names the buffer with the result.
+// hist holds nseg * 4 * 256 u32, table radix_table_words(nseg, n_cap) u32.
+hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
+                             uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *hist,
+                             uint32_t *table, int *out_index);
 
-// (d) Tile ranges: tile_start[0..num_tiles] from the tile-sorted pair keys.
+// ---- tiles.hip: (b) emission, (d) ranges ----
+uint32_t emit_blocks(uint32_t n);
+// One (global tile id, record index) pair per overlapped 16x16 tile of each
+// view's depth-ordered splats, views in order. bcnt: nviews rows of
+// emit_blocks(n) block counts; vtot: nviews entries. Sets counters[CNT_PAIRS]
+// (exact) and counters[CNT_OVERFLOW] when above pair_cap.
+hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
+                       const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, uint32_t *bcnt,
+                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap, uint32_t *counters);
+// tile_start[0..num_tiles] from the tile-sorted pair keys.
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *counters, uint64_t pair_cap,
                          uint32_t num_tiles, uint32_t *tile_start);
+// view_pairs[v] = pairs of view v (from the tile ranges).
+hipError_t launch_view_pairs(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                             const uint32_t *tile_start, uint32_t *view_pairs);
 
-// (e) Front-to-back blend of every 16x16 tile into RGB8.
-hipError_t launch_blend(hipStream_t s, const ViewCtx &v, const uint32_t *tile_start, const uint32_t *sval,
-                        const uint4 *rec, uint8_t *rgb, uint32_t *counters);
+// ---- blend.hip: (e) front-to-back blend of every tile of every view ----
+hipError_t launch_blend(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters);
 
 }  // namespace sp

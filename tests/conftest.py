@@ -8,6 +8,14 @@ ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
+# torch's wheel carries its own HIP runtime under the same soname as the
+# system one; whichever loads first serves the whole process. Loading torch
+# before libsplat_hip.so keeps one runtime for the tests that use both.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")

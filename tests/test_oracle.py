@@ -401,3 +401,24 @@ def test_stats_count_pairs(oracle):
         idx = vals[ts[t]:ts[t + 1]].astype(np.int64)
         order = np.lexsort((idx, d))
         assert np.array_equal(order, np.arange(len(order)))
+
+
+def test_fp64_reference_agrees_with_f32_and_brute_force(oracle):
+    """The FP64 yardstick of the precision variant: its projection matches the float64
+    EWA projection above and its image the f32 renderers (max error <= 2/255)."""
+    s = S.random_scene(400, seed=12, sh_degree=3, ls_lo=-3.5, ls_hi=-2.0)
+    cam = S.OrbitCamera.for_scene(s)
+    cam.rotate(0.4, 0.1)
+    v = cam.view()
+    valid, p64 = oracle.project_f64(s, v, 96, 64)
+    fv, pf, _ = oracle.project(s, v, 96, 64)
+    assert valid.sum() > 300
+    same = valid & fv
+    assert same.sum() >= valid.sum() - 2
+    rel = np.abs(p64[same][:, :5] - pf[same][:, :5]) / np.maximum(np.abs(p64[same][:, :5]), 1e-3)
+    assert rel.max() < 1e-3
+    assert np.abs(p64[same][:, 5:8] - pf[same][:, 5:8]).max() < 1e-5
+    rgb, img = oracle.render_f64(s, v, 96, 64, (0.1, 0.2, 0.3))
+    a, _ = oracle.CpuTiled(s).render(v, 96, 64, (0.1, 0.2, 0.3))
+    assert np.abs(a.astype(int) - img.astype(int)).max() <= 2
+    assert S.psnr(a, img) >= 50
