@@ -506,7 +506,7 @@ void project_group(or_cpu *h, Group &g, size_t base, size_t len, const Ctx &ctx,
 
 // raster_tile (src/render/cpu.rs:609-719). `work` counts live in-box pixel evaluations.
 void raster_tile(const RSplat *splats, const uint64_t *keys, size_t nk, uint32_t ox, uint32_t oy, uint32_t width,
-                 uint32_t height, const float bg[3], uint8_t *out, uint64_t *work) {
+                 uint32_t height, const float bg[3], uint8_t *out, uint64_t *work, size_t *processed = nullptr) {
     const uint32_t tw = std::min(TILE, width - ox);
     const uint32_t th = std::min(TILE, height - oy);
     float tr[TILE_GROUPS][LANES], cr[TILE_GROUPS][LANES], cg[TILE_GROUPS][LANES], cb[TILE_GROUPS][LANES];
@@ -522,6 +522,7 @@ void raster_tile(const RSplat *splats, const uint64_t *keys, size_t nk, uint32_t
     const float fx = (float)ox, fy = (float)oy;
     float dxs[ROW_GROUPS][LANES], qx[ROW_GROUPS][LANES];
     uint64_t evals = 0;
+    if (processed) *processed = nk;
     for (size_t ki = 0; ki < nk; ++ki) {
         const RSplat &s = splats[(uint32_t)keys[ki]];
         const uint32_t bx0 = s.bbox[0], by0 = s.bbox[1], bx1 = s.bbox[2], by1 = s.bbox[3];
@@ -575,7 +576,10 @@ void raster_tile(const RSplat *splats, const uint64_t *keys, size_t nk, uint32_t
                 if (live[ly] == 0) live_rows -= 1;
             }
         }
-        if (live_rows == 0) break;
+        if (live_rows == 0) {
+            if (processed) *processed = ki + 1;
+            break;
+        }
     }
     if (work) *work = evals;
     if (!out) return;
@@ -719,6 +723,30 @@ int or_cpu_render(or_cpu *h, const sp_view *view, uint32_t w, uint32_t hh, const
     });
     if (work)
         for (uint64_t v : works) *work += v;
+    return 0;
+}
+
+// Per-tile blend statistics of CpuTiled: lens[t] = pairs of tile t,
+// processed[t] = pairs blended before every pixel of the tile saturated.
+int or_cpu_tile_stats(or_cpu *h, const sp_view *view, uint32_t w, uint32_t hh, uint32_t *lens, uint32_t *processed,
+                      int threads) {
+    if (w == 0 || hh == 0) return 0;
+    threads = resolve_threads(threads);
+    Frame3 fr;
+    sp_stats st;
+    if (!cpu_bucket(h, view, w, hh, threads, &fr, &st)) return -1;
+    const float bg[3] = {0.f, 0.f, 0.f};
+    parallel_for(fr.num_tiles, threads, [&](size_t t) {
+        const size_t b = h->tile_start[t], e = h->tile_start[t + 1];
+        sort_hi32(h->keys.data() + b, h->sort_tmp.data() + b, e - b);
+        fix_ties(h->keys.data() + b, e - b, h->orig.data());
+        const uint32_t tx = (uint32_t)t % fr.tiles_x, ty = (uint32_t)t / fr.tiles_x;
+        size_t done = 0;
+        raster_tile(h->splats.data(), h->keys.data() + b, e - b, tx * TILE, ty * TILE, w, hh, bg, nullptr, nullptr,
+                    &done);
+        lens[t] = (uint32_t)(e - b);
+        processed[t] = (uint32_t)done;
+    });
     return 0;
 }
 
