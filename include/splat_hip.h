@@ -177,6 +177,50 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
                     const float background[3], uint8_t *const *outs, int32_t outs_on_device,
                     sp_stats *stats);
 
+/* ---------------------------------------------------------------------------
+ * Differentiable rendering (no reference counterpart: splat-tui renders only;
+ * this is the adjoint of the same forward). A handle made by sp_diff_create
+ * holds no scene: each sp_diff_forward reads raw, unactivated parameters from
+ * the caller's device memory, activates them on the device (exp of the
+ * log-scales, sigmoid of the logits, normalised quaternions, covariance,
+ * k = sqrt(2 ln(o/ALPHA_MIN))) and renders float images C + T*bg (the value
+ * to_u8 would quantise, src/render/mod.rs:282-285) with the `cpu` backend's
+ * blend rules. sp_diff_backward then turns dL/dimage into dL/dparameters.
+ * ------------------------------------------------------------------------- */
+
+/* Device pointers, row-major per gaussian (the layout of contiguous
+ * [n][k] float tensors). */
+typedef struct sp_params {
+    const float *means;          /* [n][3] */
+    const float *log_scales;     /* [n][3] */
+    const float *quats;          /* [n][4] x,y,z,w, any non-zero length */
+    const float *opacity_logits; /* [n] */
+    const float *colors;         /* [n][3] base colour (SH DC term) */
+    const float *sh_rest;        /* [n][(deg+1)^2-1][3], NULL for degree 0 */
+} sp_params;
+
+/* Device pointers receiving dL/dparameter, same shapes as sp_params (overwritten). */
+typedef struct sp_param_grads {
+    float *means, *log_scales, *quats, *opacity_logits, *colors, *sh_rest;
+} sp_param_grads;
+
+int sp_diff_create(uint64_t n, uint32_t sh_degree, const sp_options *opt, sp_handle **out);
+
+/* Render n_views views into images[v] (device, height*width*3 floats,
+ * row-major RGB). The handle keeps what the backward needs; `params` must
+ * stay valid and unchanged until the matching sp_diff_backward. */
+int sp_diff_forward(sp_handle *h, const sp_params *params, const sp_view *views, const uint32_t *wh,
+                    uint32_t n_views, const float background[3], float *const *images, sp_stats *stats);
+
+/* Gradients of a loss L given dL/dimage (grad_images[v], device, same layout
+ * as the forward's images) for the handle's last sp_diff_forward. Opacity
+ * thresholds, the bounding boxes and the depth order are treated as
+ * constants (they are piecewise constant in the parameters). The reduction
+ * over pixels and pairs uses no floating-point atomics: results are
+ * bit-reproducible. backward_ms (optional) receives the wall time. */
+int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *grad_images,
+                     const sp_param_grads *grads, double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

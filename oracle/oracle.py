@@ -235,3 +235,97 @@ def render_f64(scene, view, width, height, background, threads: int = 0):
                     rgb.ctypes.data_as(C.POINTER(C.c_double)), img.ctypes.data_as(N.u8p), threads)
     del keep
     return rgb, img
+
+
+# ---- FP64 reference of the differentiable render (oracle/diff_oracle.cpp) ----
+
+PARAM_KEYS = ("means", "log_scales", "quats", "opacity_logits", "colors", "sh_rest")
+
+
+def _diff_sigs(L):
+    if getattr(L, "_diff", False):
+        return
+    d64 = C.POINTER(C.c_double)
+    vw = C.POINTER(N.SpView)
+    L.or_diff_create.restype = C.c_void_p
+    L.or_diff_create.argtypes = [N.f32p, N.f32p, N.f32p, N.f32p, N.f32p, N.f32p, C.c_uint64, C.c_uint32, vw,
+                                 N.u32p, C.c_uint32, N.f32p, C.POINTER(N.f32p), C.c_int]
+    L.or_diff_destroy.restype = None
+    L.or_diff_destroy.argtypes = [C.c_void_p]
+    L.or_diff_loss.restype = C.c_double
+    L.or_diff_loss.argtypes = [C.c_void_p]
+    L.or_diff_image.restype = None
+    L.or_diff_image.argtypes = [C.c_void_p, C.c_uint32, d64]
+    L.or_diff_grad.restype = None
+    L.or_diff_grad.argtypes = [C.c_void_p, d64]
+    L.or_diff_fd.restype = C.c_double
+    L.or_diff_fd.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_double]
+    L.or_diff_nparams.restype = C.c_uint32
+    L.or_diff_nparams.argtypes = [C.c_void_p]
+    L._diff = True
+
+
+class DiffF64:
+    """FP64 forward, analytic gradient and central finite differences of
+    L = sum over views, pixels, channels of (image - target)^2 for raw
+    parameters (dict of float32 arrays keyed by PARAM_KEYS). targets: list of
+    float32 (h, w, 3) images or None (zero targets)."""
+
+    def __init__(self, params: dict, sh_degree: int, views, sizes, background, targets=None, threads: int = 0):
+        L = lib()
+        _diff_sigs(L)
+        self._L = L
+        self.n = len(params["means"])
+        self.deg = sh_degree
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        arrs = [np.ascontiguousarray(params[k], np.float32) for k in PARAM_KEYS]
+        if arrs[5].size == 0:
+            arrs[5] = np.zeros(1, np.float32)
+        nv = len(views)
+        varr = (N.SpView * max(nv, 1))(*views)
+        wh = np.array([x for s in self.sizes for x in s], np.uint32)
+        tg = [None if targets is None or targets[v] is None else
+              np.ascontiguousarray(targets[v], np.float32) for v in range(nv)]
+        tptr = (N.f32p * max(nv, 1))(*[N.fptr(t) if t is not None else None for t in tg])
+        self._h = L.or_diff_create(*[N.fptr(a) for a in arrs], self.n, sh_degree, varr, wh.ctypes.data_as(N.u32p),
+                                   nv, N.fptr(_bg(background)), tptr, threads)
+        self.np = int(L.or_diff_nparams(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.or_diff_destroy(h)
+            self._h = None
+
+    def loss(self) -> float:
+        return float(self._L.or_diff_loss(self._h))
+
+    def image(self, v: int) -> np.ndarray:
+        w, h = self.sizes[v]
+        out = np.zeros((h, w, 3), np.float64)
+        self._L.or_diff_image(self._h, v, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def grad_flat(self) -> np.ndarray:
+        g = np.zeros((self.n, self.np), np.float64)
+        self._L.or_diff_grad(self._h, g.ctypes.data_as(C.POINTER(C.c_double)))
+        return g
+
+    def grad(self) -> dict:
+        return split_params(self.grad_flat(), self.deg)
+
+    def fd(self, i: int, k: int, eps: float) -> float:
+        """Central difference of the loss in parameter k of splat i (flat per-splat order
+        [mean 3, log-scale 3, quat 4, logit, colour 3, sh 3*rest])."""
+        return float(self._L.or_diff_fd(self._h, i, k, eps))
+
+
+def split_params(flat: np.ndarray, deg: int) -> dict:
+    """(n, 14 + 3*rest) per-splat rows -> dict keyed by PARAM_KEYS."""
+    n = flat.shape[0]
+    rest = (deg + 1) ** 2 - 1
+    return {
+        "means": flat[:, 0:3], "log_scales": flat[:, 3:6], "quats": flat[:, 6:10],
+        "opacity_logits": flat[:, 10], "colors": flat[:, 11:14],
+        "sh_rest": flat[:, 14:].reshape(n, rest, 3),
+    }

@@ -119,6 +119,30 @@ class SpFrameInfo(C.Structure):
     ]
 
 
+class SpParams(C.Structure):
+    """Device pointers of the raw parameters (sp_params)."""
+    _fields_ = [
+        ("means", C.c_void_p),
+        ("log_scales", C.c_void_p),
+        ("quats", C.c_void_p),
+        ("opacity_logits", C.c_void_p),
+        ("colors", C.c_void_p),
+        ("sh_rest", C.c_void_p),
+    ]
+
+
+class SpParamGrads(C.Structure):
+    """Device pointers receiving the parameter gradients (sp_param_grads)."""
+    _fields_ = [
+        ("means", C.c_void_p),
+        ("log_scales", C.c_void_p),
+        ("quats", C.c_void_p),
+        ("opacity_logits", C.c_void_p),
+        ("colors", C.c_void_p),
+        ("sh_rest", C.c_void_p),
+    ]
+
+
 # Every symbol include/splat_hip.h declares, with its signature.
 HIP_SYMBOLS = {
     "sp_default_options": (None, [C.POINTER(SpOptions)]),
@@ -140,6 +164,16 @@ HIP_SYMBOLS = {
     "sp_render_batch": (
         C.c_int,
         [C.c_void_p, C.POINTER(SpView), u32p, C.c_uint32, f32p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SpStats)],
+    ),
+    "sp_diff_create": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(SpOptions), C.POINTER(C.c_void_p)]),
+    "sp_diff_forward": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpParams), C.POINTER(SpView), u32p, C.c_uint32, f32p, C.POINTER(C.c_void_p),
+         C.POINTER(SpStats)],
+    ),
+    "sp_diff_backward": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpParams), C.POINTER(C.c_void_p), C.POINTER(SpParamGrads), C.POINTER(C.c_double)],
     ),
 }
 

@@ -83,6 +83,11 @@ ViewCtx make_ctx(const sp_view &v, uint32_t w, uint32_t h, const float bg[3]) {
     return c;
 }
 
+// What a batch's blend stage produces.
+constexpr int BL_NONE = 0;  // stop after the tile ranges
+constexpr int BL_RGB = 1;   // RGB8 images (Renderer::render)
+constexpr int BL_DIFF = 2;  // float images plus the state of the adjoint
+
 uint32_t bits_for(uint32_t max_value) { return max_value == 0 ? 0u : 32u - (uint32_t)__builtin_clz(max_value); }
 
 double now_ms() {
@@ -135,6 +140,24 @@ struct sp_handle {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     sp_frame_info last{};
     int pair_buf = 0;
+    int depth_buf = 0;  // ping-pong buffer holding the last depth sort's output
+    int blend_batch = 256;  // records staged through LDS per blend step (64, 128 or 256)
+    // Differentiable path (sp_diff_create): raw parameters on the caller's
+    // device memory, per-pair owners, per-pixel blend state, gradient scratch.
+    bool diff = false;
+    sp::DevParams prm{};
+    uint32_t *d_powner = nullptr;  // pair_cap: record index per emission index
+    float *d_pixT = nullptr;       // pix_cap: final transmittance per pixel (256 per tile)
+    uint32_t *d_pixlast = nullptr; // pix_cap: tile-list position after the last contribution
+    uint64_t pix_cap = 0;
+    float *d_gpair = nullptr;  // NG2D x gpair_cap per-pair gradients
+    uint64_t gpair_cap = 0;
+    float *d_g2d = nullptr;  // NG2D x g2d_cap per-record gradients
+    uint64_t g2d_cap = 0;
+    float *d_part = nullptr;  // segmented-reduction carries
+    uint64_t part_cap = 0;
+    uint32_t fwd_views = 0, fwd_tiles = 0;  // shape of the last differentiable forward
+    uint64_t fwd_pairs = 0;
     // Captured batch graph.
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -199,15 +222,18 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
         dfree(h, h->d_pkey[b], h->pair_cap * 4);
         dfree(h, h->d_pval[b], h->pair_cap * 4);
     }
+    dfree(h, h->d_powner, h->pair_cap * 4);
     h->pair_cap = 0;
     for (int b = 0; b < 2; ++b) {
         int rc = dalloc(h, (void **)&h->d_pkey[b], cap * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_pval[b], cap * 4);
+        if (!rc && b == 0 && h->diff) rc = dalloc(h, (void **)&h->d_powner, cap * 4);
         if (rc) {
             for (int c = 0; c < 2; ++c) {
                 dfree(h, h->d_pkey[c], cap * 4);
                 dfree(h, h->d_pval[c], cap * 4);
             }
+            dfree(h, h->d_powner, cap * 4);
             return rc;
         }
     }
@@ -263,6 +289,48 @@ int ensure_tiles(sp_handle *h, uint64_t tiles) {
     return SP_OK;
 }
 
+// Per-pixel blend state of the differentiable forward (256 entries per tile).
+int ensure_pix(sp_handle *h, uint64_t tiles) {
+    const uint64_t need = tiles * sp::BLEND_THREADS;
+    if (need <= h->pix_cap) return SP_OK;
+    dfree(h, h->d_pixT, h->pix_cap * 4);
+    dfree(h, h->d_pixlast, h->pix_cap * 4);
+    h->pix_cap = 0;
+    int rc = dalloc(h, (void **)&h->d_pixT, need * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_pixlast, need * 4);
+    if (rc) return rc;
+    h->pix_cap = need;
+    return SP_OK;
+}
+
+// Gradient scratch of the backward: per-pair, per-record and carry arrays.
+int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
+    int rc = SP_OK;
+    if (pairs > h->gpair_cap) {
+        dfree(h, h->d_gpair, h->gpair_cap * sp::NG2D * 4);
+        h->gpair_cap = 0;
+        rc = dalloc(h, (void **)&h->d_gpair, pairs * sp::NG2D * 4);
+        if (rc) return rc;
+        h->gpair_cap = pairs;
+    }
+    if (records > h->g2d_cap) {
+        dfree(h, h->d_g2d, h->g2d_cap * sp::NG2D * 4);
+        h->g2d_cap = 0;
+        rc = dalloc(h, (void **)&h->d_g2d, records * sp::NG2D * 4);
+        if (rc) return rc;
+        h->g2d_cap = records;
+    }
+    const uint64_t parts = sp::seg_blocks(pairs) * 2 * sp::NG2D;
+    if (parts > h->part_cap) {
+        dfree(h, h->d_part, h->part_cap * 4);
+        h->part_cap = 0;
+        rc = dalloc(h, (void **)&h->d_part, parts * 4);
+        if (rc) return rc;
+        h->part_cap = parts;
+    }
+    return SP_OK;
+}
+
 int ensure_rgb(sp_handle *h, uint64_t bytes) {
     if (bytes <= h->rgb_cap) return SP_OK;
     dfree(h, h->d_rgb, h->rgb_cap);
@@ -281,7 +349,8 @@ sp::DevScene dev_scene(const sp_handle *h) {
     s.n = h->n;
     s.deg = h->deg;
     s.rest = h->rest;
-    s.storage = (uint32_t)h->opt.storage;
+    s.storage = h->diff ? (uint32_t)sp::STORE_PARAMS : (uint32_t)h->opt.storage;
+    s.prm = h->prm;
     return s;
 }
 
@@ -299,7 +368,7 @@ int setup(sp_handle *h) {
 
 // Enqueue the whole pipeline for the first `nviews` entries of h_views. The
 // sorted pairs end in ping-pong buffer *pair_buf.
-int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, bool blend, bool events,
+int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, int mode, bool events,
             int *pair_buf) {
     const sp::DevScene sc = dev_scene(h);
     hipStream_t s = h->stream;
@@ -313,8 +382,10 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
                                h->d_dval[0], nullptr, nullptr, nullptr));
     int di = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_hist, h->d_table, &di));
+    h->depth_buf = di;
     HIP_TRY(sp::launch_emit(s, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, h->d_bcnt, vtot,
-                            h->d_pkey[0], h->d_pval[0], h->pair_cap, h->d_counters));
+                            h->d_pkey[0], h->d_pval[0], mode == BL_DIFF ? h->d_powner : nullptr, h->pair_cap,
+                            h->d_counters));
     if (events) HIP_TRY(hipEventRecord(h->ev[1], s));
     int pi = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
@@ -322,9 +393,13 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters, h->pair_cap, total_tiles, h->d_tile_start));
     HIP_TRY(sp::launch_view_pairs(s, h->d_views, nviews, total_tiles, h->d_tile_start, view_pairs));
     if (events) HIP_TRY(hipEventRecord(h->ev[2], s));
-    if (blend)
+    if (mode == BL_RGB)
         HIP_TRY(sp::launch_blend(s, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi], h->d_rec,
-                                 h->d_counters));
+                                 h->d_counters, h->blend_batch));
+    if (mode == BL_DIFF)
+        HIP_TRY(sp::launch_blend_diff(s, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
+                                      h->d_powner, h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast,
+                                      h->blend_batch));
     if (events) HIP_TRY(hipEventRecord(h->ev[3], s));
     HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, sp::CNT_SLOTS * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS, vis, nviews * 4, hipMemcpyDeviceToHost, s));
@@ -333,10 +408,12 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     return SP_OK;
 }
 
-// Render `nviews` views (outs[v] device RGB8 buffers; blend == false stops
-// after the tile ranges). use_graph: capture the batch once and replay it.
+// Render `nviews` views: mode BL_RGB writes outs[v] as device RGB8 buffers,
+// BL_DIFF as device float images (the differentiable forward), BL_NONE stops
+// after the tile ranges. use_graph: capture the batch once and replay it.
 int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_t *wh, const float bg[3],
-              uint8_t *const *outs, bool blend, bool use_graph, sp_stats *stats) {
+              void *const *outs, int mode, bool use_graph, sp_stats *stats) {
+    const bool blend = mode != BL_NONE;
     const double t0 = now_ms();
     if (nviews == 0) return SP_OK;
     if (nviews > sp::MAX_VIEWS) return fail(SP_E_INVALID, "too many views in one batch");
@@ -350,12 +427,14 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         std::memset(&p, 0, sizeof p);
         p.ctx = make_ctx(views[v], wh[2 * v], wh[2 * v + 1], bg);
         p.tile_base = (uint32_t)total;
-        p.out = outs ? outs[v] : nullptr;
+        if (outs && mode == BL_RGB) p.out = (uint8_t *)outs[v];
+        if (outs && mode == BL_DIFF) p.fimg = (float *)outs[v];
         total += (uint64_t)p.ctx.tiles_x * p.ctx.tiles_y;
     }
     if (total >= 0xFFFFFFFFull) return fail(SP_E_INVALID, "too many tiles in one batch");
     const uint32_t total_tiles = (uint32_t)total;
     rc = ensure_tiles(h, total_tiles);
+    if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
     if (rc) return rc;
     const int tile_bits = (int)bits_for(total_tiles ? total_tiles - 1 : 0);
     const bool events = h->opt.timing && !use_graph;
@@ -366,12 +445,12 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             key.nviews = nviews;
             key.total_tiles = total_tiles;
             key.tile_bits = (uint32_t)tile_bits;
-            key.with_blend = blend ? 1u : 0u;
+            key.with_blend = (uint32_t)mode;
             key.generation = h->generation;
             if (!h->gexec || !(key == h->gkey)) {
                 drop_graph(h);
                 HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                rc = enqueue(h, nviews, total_tiles, tile_bits, blend, false, &h->graph_pair_buf);
+                rc = enqueue(h, nviews, total_tiles, tile_bits, mode, false, &h->graph_pair_buf);
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(h->stream, &g);
                 if (rc) {
@@ -386,7 +465,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
             pb = h->graph_pair_buf;
         } else {
-            rc = enqueue(h, nviews, total_tiles, tile_bits, blend, events, &pb);
+            rc = enqueue(h, nviews, total_tiles, tile_bits, mode, events, &pb);
             if (rc) return rc;
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
@@ -437,8 +516,8 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
 int render_one(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, const float bg[3],
                uint8_t *d_out, sp_stats *stats) {
     const uint32_t wh[2] = {width, height};
-    uint8_t *outs[1] = {d_out};
-    return run_views(h, 1, view, wh, bg, outs, d_out != nullptr, false, stats);
+    void *outs[1] = {d_out};
+    return run_views(h, 1, view, wh, bg, outs, d_out != nullptr ? BL_RGB : BL_NONE, false, stats);
 }
 
 // Upload the scene in the handle's storage format.
@@ -492,6 +571,39 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     return SP_OK;
 }
 
+// Device, stream and timing events of a new handle.
+int open_handle(sp_handle *h) {
+    int rc = setup(h);
+    if (rc) return rc;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(SP_E_DEVICE, "hipStreamCreate failed");
+    for (auto &e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(SP_E_DEVICE, "hipEventCreate failed");
+    return SP_OK;
+}
+
+// Counters, one view's arrays and the initial pair capacity.
+int finish_handle(sp_handle *h) {
+    int rc = dalloc(h, (void **)&h->d_counters, sp::CNT_SLOTS * 4);
+    if (rc) return rc;
+    const uint32_t zero[sp::CNT_SLOTS] = {0};
+    if (hipMemcpyAsync(h->d_counters, zero, sizeof zero, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(SP_E_DEVICE, "counter init failed");
+    rc = ensure_views(h, 1);
+    const uint64_t cap0 =
+        h->opt.initial_pairs ? h->opt.initial_pairs : std::max<uint64_t>(16ull * h->n, 1ull << 20);
+    if (!rc) rc = ensure_pairs(h, cap0);
+    return rc;
+}
+
+int device_ok(const sp_options &opt) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SP_E_DEVICE, "no HIP device");
+    if (opt.device < 0 || opt.device >= count) return fail(SP_E_DEVICE, "device ordinal out of range");
+    return SP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -529,6 +641,12 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_rgb);
     (void)hipFree(h->d_counters);
     (void)hipFree(h->d_hist);
+    (void)hipFree(h->d_powner);
+    (void)hipFree(h->d_pixT);
+    (void)hipFree(h->d_pixlast);
+    (void)hipFree(h->d_gpair);
+    (void)hipFree(h->d_g2d);
+    (void)hipFree(h->d_part);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->h_views) (void)hipHostFree(h->h_views);
     for (auto &e : h->ev)
@@ -558,9 +676,7 @@ int sp_create(const sp_scene *scene, const sp_options *opt_in, sp_handle **out) 
     if (n && opt.storage != SP_STORE_F32 && (!scene->scales || !scene->rotations))
         return fail(SP_E_INVALID, "16-bit storage needs scales and rotations");
     if (n && rest && !scene->sh_rest) return fail(SP_E_INVALID, "sh_rest missing for sh_degree > 0");
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SP_E_DEVICE, "no HIP device");
-    if (opt.device < 0 || opt.device >= count) return fail(SP_E_DEVICE, "device ordinal out of range");
+    if (device_ok(opt)) return SP_E_DEVICE;
 
     sp_handle *h = new sp_handle();
     h->device = opt.device;
@@ -572,24 +688,12 @@ int sp_create(const sp_scene *scene, const sp_options *opt_in, sp_handle **out) 
         sp_destroy(h);
         return rc;
     };
-    int rc = setup(h);
+    int rc = open_handle(h);
     if (rc) return bail(rc);
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(SP_E_DEVICE, "hipStreamCreate failed"));
-    for (auto &e : h->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(SP_E_DEVICE, "hipEventCreate failed"));
     // Structure-of-arrays layout (one 64-aligned array per field).
     h->stride = std::max<uint64_t>(((uint64_t)n + 63) / 64 * 64, 64);
     rc = upload_scene(h, scene);
-    if (!rc) rc = dalloc(h, (void **)&h->d_counters, sp::CNT_SLOTS * 4);
-    if (rc) return bail(rc);
-    const uint32_t zero[sp::CNT_SLOTS] = {0};
-    if (hipMemcpyAsync(h->d_counters, zero, sizeof zero, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-        return bail(fail(SP_E_DEVICE, "counter init failed"));
-    rc = ensure_views(h, 1);
-    const uint64_t cap0 = opt.initial_pairs ? opt.initial_pairs : std::max<uint64_t>(16ull * n, 1ull << 20);
-    if (!rc) rc = ensure_pairs(h, cap0);
+    if (!rc) rc = finish_handle(h);
     if (rc) return bail(rc);
     *out = h;
     return SP_OK;
@@ -686,11 +790,15 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     if (np == 0) return SP_OK;
     if (!keys || !vals) return fail(SP_E_INVALID, "sp_debug_pairs: NULL keys/vals");
     std::vector<uint32_t> tk(np);
-    std::vector<uint4> rec((uint64_t)h->n * 3);
     HIP_TRY(hipMemcpy(tk.data(), h->d_pkey[h->pair_buf], np * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vals, h->d_pval[h->pair_buf], np * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(rec.data(), h->d_rec, rec.size() * 16, hipMemcpyDeviceToHost));
-    for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | rec[3ull * vals[p] + 2].w;
+    // Depth bits of each visible splat, from the depth sort's (depth, index) output.
+    const uint32_t nv = st.visible;
+    std::vector<uint32_t> dk(nv), dv(nv), depth(h->n, 0u);
+    HIP_TRY(hipMemcpy(dk.data(), h->d_dkey[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(dv.data(), h->d_dval[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
+    for (uint32_t q = 0; q < nv; ++q) depth[dv[q]] = dk[q];
+    for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | depth[vals[p]];
     return SP_OK;
 }
 
@@ -740,7 +848,8 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
         }
     }
     std::vector<sp_stats> st(nb);
-    rc = run_views(h, nb, bv.data(), bwh.data(), background, douts.data(), true, true, st.data());
+    std::vector<void *> vouts(douts.begin(), douts.end());
+    rc = run_views(h, nb, bv.data(), bwh.data(), background, vouts.data(), BL_RGB, true, st.data());
     if (rc) return rc;
     if (!outs_on_device) {
         for (uint32_t k = 0; k < nb; ++k)
@@ -748,6 +857,99 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
     }
     if (stats)
         for (uint32_t k = 0; k < nb; ++k) stats[idx[k]] = st[k];
+    return SP_OK;
+}
+
+int sp_diff_create(uint64_t n, uint32_t sh_degree, const sp_options *opt_in, sp_handle **out) {
+    if (!out) return fail(SP_E_INVALID, "sp_diff_create: out is NULL");
+    *out = nullptr;
+    sp_options opt;
+    if (opt_in)
+        opt = *opt_in;
+    else
+        sp_default_options(&opt);
+    if (sh_degree > 3) return fail(SP_E_INVALID, "sh_degree must be 0..3");
+    if (n > 0x7FFFFFFFull) return fail(SP_E_INVALID, "more than 2^31 gaussians");
+    if (device_ok(opt)) return SP_E_DEVICE;
+    sp_handle *h = new sp_handle();
+    h->device = opt.device;
+    h->opt = opt;
+    h->opt.storage = SP_STORE_F32;
+    h->diff = true;
+    h->n = (uint32_t)n;
+    h->deg = sh_degree;
+    h->rest = (sh_degree + 1) * (sh_degree + 1) - 1;
+    h->stride = std::max<uint64_t>((n + 63) / 64 * 64, 64);
+    int rc = open_handle(h);
+    if (!rc) rc = finish_handle(h);
+    if (rc) {
+        sp_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return SP_OK;
+}
+
+int sp_diff_forward(sp_handle *h, const sp_params *params, const sp_view *views, const uint32_t *wh,
+                    uint32_t n_views, const float background[3], float *const *images, sp_stats *stats) {
+    if (!h || !params || !views || !wh || !background || !images)
+        return fail(SP_E_INVALID, "sp_diff_forward: NULL argument");
+    if (!h->diff) return fail(SP_E_INVALID, "sp_diff_forward: handle not made by sp_diff_create");
+    if (h->n && (!params->means || !params->log_scales || !params->quats || !params->opacity_logits ||
+                 !params->colors || (h->rest && !params->sh_rest)))
+        return fail(SP_E_INVALID, "sp_diff_forward: parameter array missing");
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const int rc = check_frame(wh[2 * v], wh[2 * v + 1]);
+        if (rc) return rc;
+        if (!images[v] && wh[2 * v] && wh[2 * v + 1]) return fail(SP_E_INVALID, "sp_diff_forward: NULL image");
+    }
+    int rc = setup(h);
+    if (rc) return rc;
+    h->prm = sp::DevParams{params->means, params->log_scales, params->quats, params->opacity_logits,
+                           params->colors, params->sh_rest};
+    h->fwd_views = 0;
+    std::vector<void *> outs(images, images + n_views);
+    rc = run_views(h, n_views, views, wh, background, outs.data(), BL_DIFF, false, stats);
+    if (rc) return rc;
+    h->fwd_views = n_views;
+    h->fwd_tiles = (uint32_t)h->last.tiles;
+    h->fwd_pairs = h->last.pairs;
+    return SP_OK;
+}
+
+int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *grad_images,
+                     const sp_param_grads *grads, double *backward_ms) {
+    if (!h || !params || !grad_images || !grads) return fail(SP_E_INVALID, "sp_diff_backward: NULL argument");
+    if (!h->diff) return fail(SP_E_INVALID, "sp_diff_backward: handle not made by sp_diff_create");
+    if (h->fwd_views == 0) return fail(SP_E_INVALID, "sp_diff_backward: no preceding sp_diff_forward");
+    if (h->n && (!grads->means || !grads->log_scales || !grads->quats || !grads->opacity_logits ||
+                 !grads->colors || (h->rest && !grads->sh_rest)))
+        return fail(SP_E_INVALID, "sp_diff_backward: gradient array missing");
+    const uint32_t nv = h->fwd_views;
+    for (uint32_t v = 0; v < nv; ++v)
+        if (!grad_images[v] && h->h_views[v].ctx.w && h->h_views[v].ctx.h)
+            return fail(SP_E_INVALID, "sp_diff_backward: NULL gradient image");
+    int rc = setup(h);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    h->prm = sp::DevParams{params->means, params->log_scales, params->quats, params->opacity_logits,
+                           params->colors, params->sh_rest};
+    const uint64_t np = h->fwd_pairs, records = (uint64_t)nv * h->n;
+    rc = ensure_grad(h, std::max<uint64_t>(np, 1), std::max<uint64_t>(records, 1));
+    if (rc) return rc;
+    for (uint32_t v = 0; v < nv; ++v) h->h_views[v].gimg = grad_images[v];
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nv * sizeof(ViewParams), hipMemcpyHostToDevice, s));
+    HIP_TRY(sp::launch_fill_f32(s, h->d_gpair, h->gpair_cap * sp::NG2D, 0.0f));
+    HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, h->g2d_cap * sp::NG2D, 0.0f));
+    HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->d_tile_start, h->d_pval[h->pair_buf],
+                                 h->d_powner, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap));
+    HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_powner, np, h->d_g2d, h->g2d_cap, h->d_part));
+    HIP_TRY(sp::launch_project_bwd(s, dev_scene(h), h->d_views, nv, h->d_g2d, h->g2d_cap, grads->means,
+                                   grads->log_scales, grads->quats, grads->opacity_logits, grads->colors,
+                                   grads->sh_rest));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (backward_ms) *backward_ms = now_ms() - t0;
     return SP_OK;
 }
 

@@ -1,0 +1,533 @@
+// Differentiation: the adjoint of the blend and of the projection.
+//
+// The differentiable forward renders float images (C + T*bg) with the same
+// pipeline as the plain path, except that the tile sort carries each pair's
+// emission index e (pairs are emitted splat by splat in depth order, so the
+// pairs of one (view, splat) record are contiguous in e) and the blend keeps,
+// per pixel, its final transmittance and the tile-list position after its
+// last contribution. The backward is three passes, none of which uses a
+// floating-point atomic, so gradients are deterministic:
+//
+//   1. k_blend_bwd: one workgroup per tile walks the tile's list back to
+//      front from the last position any of its pixels used, recovering each
+//      pixel's transmittance before a contribution as T_i = T_{i+1} / (1 - a_i)
+//      and carrying the colour accumulated behind it, S. Per pixel and record:
+//        dL/dc     = G * a * T_i
+//        dL/da     = G . (c * T_i - S / (1 - a))
+//        dL/dop    = dL/da * exp(pw),   dL/dpw = dL/da * a   (0 where a is capped)
+//        dL/dmean2 = dL/dpw * (A dx + B dy, B dx + C dy)
+//        dL/dconic = dL/dpw * (-dx^2 / 2, -dx dy, -dy^2 / 2)
+//      with G = dL/dpixel. The nine values are summed over the tile's 256
+//      pixels in a fixed order (wave butterflies, then the four waves) and
+//      stored per pair at its emission index.
+//   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
+//      sums each (view, splat) record's pairs: segments inside a 256-pair
+//      block are summed sequentially from LDS; a segment that crosses blocks
+//      is finished by its first block, which adds the partial sums its
+//      successors left.
+//   3. k_project_bwd: one thread per splat chains the per-view 2D gradients
+//      through the EWA projection, the covariance (scale, rotation), the
+//      opacity sigmoid and the SH colour, and sums the views.
+#include "sp_launch.h"
+#include "sp_splat.h"
+
+namespace sp {
+namespace {
+
+constexpr int XCDS = 8;
+constexpr int BWD_BATCH = 64;  // records per LDS step of the adjoint blend
+constexpr int SR_THREADS = 256;
+constexpr int PBWD_THREADS = 128;
+
+__global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
+    const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
+    const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ powner,
+    const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
+    float *__restrict__ gpair, uint64_t gstride, uint32_t per_xcd) {
+    __shared__ uint4 s_rec[BWD_BATCH][3];
+    __shared__ uint32_t s_e[BWD_BATCH];
+    __shared__ float s_part[4][NG2D][BWD_BATCH];
+    __shared__ uint32_t s_wmax[4];
+    const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (gt >= total_tiles) return;
+    const uint32_t vi = view_of_tile(views, nviews, gt);
+    const ViewCtx &v = views[vi].ctx;
+    const uint32_t t = gt - views[vi].tile_base;
+    const uint32_t tx = t % v.tiles_x, ty = t / v.tiles_x;
+    const uint32_t ox = tx * TILE, oy = ty * TILE;
+    // Same pixel <-> thread mapping as the forward blend (8x8 quadrant per wave).
+    const uint32_t w = threadIdx.x >> 6, lane_i = lane_id();
+    const uint32_t qx0 = ox + 8u * (w & 1u), qy0 = oy + 8u * (w >> 1);
+    const uint32_t px = qx0 + (lane_i & 7u), py = qy0 + (lane_i >> 3);
+    const uint32_t lx = px - ox, ly = py - oy;
+    const bool inside = px < v.w && py < v.h;
+    const float fx = (float)ox;
+    const float gx = (float)(lx & ~3u);
+    const float lane = (float)(lx & 3u);
+    const float pyc = ((float)oy + (float)ly) + 0.5f;
+    const uint64_t g = (uint64_t)gt * BLEND_THREADS + threadIdx.x;
+    float T = pix_T[g];
+    const uint32_t last = pix_last[g];
+    float G0 = 0.0f, G1 = 0.0f, G2 = 0.0f;
+    if (inside) {
+        const float *gi = views[vi].gimg + 3 * ((uint64_t)py * v.w + px);
+        G0 = gi[0];
+        G1 = gi[1];
+        G2 = gi[2];
+    }
+    float S0 = T * v.bg[0], S1 = T * v.bg[1], S2 = T * v.bg[2];
+    uint32_t m = last;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if (lane_i == 0) s_wmax[w] = m;
+    __syncthreads();
+    const uint32_t lim = max(max(s_wmax[0], s_wmax[1]), max(s_wmax[2], s_wmax[3]));
+    const uint32_t start = tile_start[gt];
+    float *part = &s_part[0][0][0];
+    for (uint32_t hi = lim; hi > 0;) {
+        const uint32_t lo = hi > (uint32_t)BWD_BATCH ? hi - BWD_BATCH : 0u;
+        const uint32_t nb = hi - lo;
+        if (threadIdx.x < nb) {
+            const uint32_t e = sval[start + lo + threadIdx.x];
+            s_e[threadIdx.x] = e;
+            const uint4 *r = rec + 3 * (uint64_t)powner[e];
+            s_rec[threadIdx.x][0] = r[0];
+            s_rec[threadIdx.x][1] = r[1];
+            s_rec[threadIdx.x][2] = r[2];
+        }
+        for (uint32_t z = threadIdx.x; z < 4u * NG2D * BWD_BATCH; z += BLEND_THREADS) part[z] = 0.0f;
+        __syncthreads();
+        for (int k = (int)nb - 1; k >= 0; --k) {
+            const uint32_t pos = lo + (uint32_t)k;
+            const uint4 r2 = s_rec[k][2];
+            const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
+            if (bx1 <= qx0 || bx0 >= qx0 + 8u || by1 <= qy0 || by0 >= qy0 + 8u) continue;  // wave-uniform
+            const bool act = pos < last && px >= bx0 && px < bx1 && py >= by0 && py < by1;
+            if (!__any(act)) continue;
+            const uint4 r0 = s_rec[k][0];
+            const uint4 r1 = s_rec[k][1];
+            const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
+            const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w);
+            const float nc = __uint_as_float(r1.x);
+            // The forward's exact arithmetic, so every mask decision repeats.
+            const float cx0 = (fx + 0.5f) - sx;
+            const float dx = lane + (gx + cx0);
+            const float dy = pyc - sy;
+            const float qx = (dx * dx) * na;
+            const float qy = (nc * dy) * dy;
+            const float bdy = nb2 * dy;
+            const float pw = (qx + qy) + bdy * dx;
+            if (!__any(act && pw >= __uint_as_float(r2.w))) continue;
+            const float op = __uint_as_float(r1.y);
+            const float ex = exp_neg(pw);
+            const float raw = ex * op;
+            const float alpha = fminf(raw, ALPHA_MAX);
+            const bool con = act && pw <= 0.0f && alpha >= ALPHA_MIN;
+            float gv[NG2D];
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c) gv[c] = 0.0f;
+            if (con) {
+                const float om = 1.0f - alpha;
+                const float Ti = T / om;
+                const float c0 = __uint_as_float(r1.z), c1 = __uint_as_float(r1.w), c2 = __uint_as_float(r2.x);
+                const float aT = alpha * Ti;
+                gv[6] = G0 * aT;
+                gv[7] = G1 * aT;
+                gv[8] = G2 * aT;
+                const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) / om;
+                S0 += c0 * aT;
+                S1 += c1 * aT;
+                S2 += c2 * aT;
+                T = Ti;
+                if (raw < ALPHA_MAX) {
+                    gv[5] = ga * ex;
+                    const float gpw = ga * alpha;
+                    const float A = -2.0f * na, B = -nb2, C = -2.0f * nc;
+                    gv[0] = gpw * (A * dx + B * dy);
+                    gv[1] = gpw * (B * dx + C * dy);
+                    gv[2] = -0.5f * gpw * (dx * dx);
+                    gv[3] = -gpw * (dx * dy);
+                    gv[4] = -0.5f * gpw * (dy * dy);
+                }
+            }
+            if (__any(con)) {
+#pragma unroll
+                for (int c = 0; c < NG2D; ++c) {
+                    const float sum = wave_sum_f(gv[c]);
+                    if (lane_i == 0) s_part[w][c][k] = sum;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < nb) {
+            const uint32_t e = s_e[threadIdx.x];
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c)
+                gpair[c * gstride + e] = (s_part[0][c][threadIdx.x] + s_part[1][c][threadIdx.x]) +
+                                         (s_part[2][c][threadIdx.x] + s_part[3][c][threadIdx.x]);
+        }
+        __syncthreads();
+        hi = lo;
+    }
+}
+
+// Segmented sum over emission order, segments = runs of equal owner (record
+// index). part holds, per block, the partial sum of a segment continuing from
+// the previous block (head, slot 0) and of one continuing into the next
+// (tail, slot 1); a block covered by one segment stores its sum as head.
+__global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restrict__ gpair, uint64_t gstride,
+                                                           const uint32_t *__restrict__ powner, uint64_t np,
+                                                           float *__restrict__ g2d, uint64_t g2d_stride,
+                                                           float *__restrict__ part) {
+    __shared__ float s_v[NG2D][SR_THREADS];
+    __shared__ uint32_t s_o[SR_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SR_THREADS;
+    const uint32_t cnt = (uint32_t)min((uint64_t)SR_THREADS, np - base);
+    const uint32_t t = threadIdx.x;
+    if (t < cnt) {
+        s_o[t] = powner[base + t];
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) s_v[c][t] = gpair[c * gstride + base + t];
+    }
+    __syncthreads();
+    if (t >= cnt) return;
+    const uint32_t o = s_o[t];
+    if (t > 0 && s_o[t - 1] == o) return;  // not the first element of its segment in this block
+    uint32_t u = t + 1;
+    while (u < cnt && s_o[u] == o) ++u;
+    float sum[NG2D];
+#pragma unroll
+    for (int c = 0; c < NG2D; ++c) sum[c] = 0.0f;
+    for (uint32_t j = t; j < u; ++j)
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) sum[c] += s_v[c][j];
+    const bool before = t == 0 && base > 0 && powner[base - 1] == o;
+    const bool after = u == cnt && base + cnt < np && powner[base + cnt] == o;
+    if (!before && !after) {
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) g2d[c * g2d_stride + o] = sum[c];
+    } else {
+        float *dst = part + ((uint64_t)blockIdx.x * 2 + (before ? 0 : 1)) * NG2D;
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) dst[c] = sum[c];
+    }
+}
+
+// Finishes segments that cross blocks: the block where such a segment starts
+// adds its tail partial and the head partials of the blocks it runs into.
+__global__ __launch_bounds__(256) void k_seg_carry(const uint32_t *__restrict__ powner, uint64_t np,
+                                                   uint32_t nblocks, const float *__restrict__ part,
+                                                   float *__restrict__ g2d, uint64_t g2d_stride) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nblocks) return;
+    const uint64_t base = (uint64_t)b * SR_THREADS;
+    const uint64_t end = min(base + SR_THREADS, np);
+    const uint32_t o = powner[end - 1];
+    if (end >= np || powner[end] != o) return;  // the block's last segment ends inside it
+    if (powner[base] == o && base > 0 && powner[base - 1] == o) return;  // started in an earlier block
+    float sum[NG2D];
+#pragma unroll
+    for (int c = 0; c < NG2D; ++c) sum[c] = part[((uint64_t)b * 2 + 1) * NG2D + c];
+    for (uint32_t b2 = b + 1; b2 < nblocks; ++b2) {
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) sum[c] += part[((uint64_t)b2 * 2) * NG2D + c];
+        const uint64_t base2 = (uint64_t)b2 * SR_THREADS;
+        const uint64_t end2 = min(base2 + SR_THREADS, np);
+        if (powner[end2 - 1] != o || end2 >= np || powner[end2] != o) break;
+    }
+#pragma unroll
+    for (int c = 0; c < NG2D; ++c) g2d[c * g2d_stride + o] = sum[c];
+}
+
+// Real SH basis of degree DEG (the order sh_color evaluates it) with its
+// gradient: y[k] and dy[k] = d y[k] / d(x, y, z).
+template <int DEG>
+__device__ __forceinline__ void sh_basis(float x, float y, float z, float *yk, float (*dy)[3]) {
+    const float c1 = SH_C1;
+    yk[0] = -c1 * y; dy[0][0] = 0.0f; dy[0][1] = -c1; dy[0][2] = 0.0f;
+    yk[1] = c1 * z;  dy[1][0] = 0.0f; dy[1][1] = 0.0f; dy[1][2] = c1;
+    yk[2] = -c1 * x; dy[2][0] = -c1; dy[2][1] = 0.0f; dy[2][2] = 0.0f;
+    if constexpr (DEG >= 2) {
+        const float xx = x * x, yy = y * y, zz = z * z;
+        const float *C2 = SH_C2;
+        yk[3] = C2[0] * x * y; dy[3][0] = C2[0] * y; dy[3][1] = C2[0] * x; dy[3][2] = 0.0f;
+        yk[4] = C2[1] * y * z; dy[4][0] = 0.0f; dy[4][1] = C2[1] * z; dy[4][2] = C2[1] * y;
+        yk[5] = C2[2] * (2.0f * zz - xx - yy);
+        dy[5][0] = -2.0f * C2[2] * x; dy[5][1] = -2.0f * C2[2] * y; dy[5][2] = 4.0f * C2[2] * z;
+        yk[6] = C2[3] * x * z; dy[6][0] = C2[3] * z; dy[6][1] = 0.0f; dy[6][2] = C2[3] * x;
+        yk[7] = C2[4] * (xx - yy); dy[7][0] = 2.0f * C2[4] * x; dy[7][1] = -2.0f * C2[4] * y; dy[7][2] = 0.0f;
+        if constexpr (DEG >= 3) {
+            const float *C3 = SH_C3;
+            yk[8] = C3[0] * y * (3.0f * xx - yy);
+            dy[8][0] = 6.0f * C3[0] * x * y; dy[8][1] = C3[0] * (3.0f * xx - 3.0f * yy); dy[8][2] = 0.0f;
+            yk[9] = C3[1] * x * y * z;
+            dy[9][0] = C3[1] * y * z; dy[9][1] = C3[1] * x * z; dy[9][2] = C3[1] * x * y;
+            yk[10] = C3[2] * y * (4.0f * zz - xx - yy);
+            dy[10][0] = -2.0f * C3[2] * x * y; dy[10][1] = C3[2] * (4.0f * zz - xx - 3.0f * yy);
+            dy[10][2] = 8.0f * C3[2] * y * z;
+            yk[11] = C3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+            dy[11][0] = -6.0f * C3[3] * x * z; dy[11][1] = -6.0f * C3[3] * y * z;
+            dy[11][2] = C3[3] * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+            yk[12] = C3[4] * x * (4.0f * zz - xx - yy);
+            dy[12][0] = C3[4] * (4.0f * zz - 3.0f * xx - yy); dy[12][1] = -2.0f * C3[4] * x * y;
+            dy[12][2] = 8.0f * C3[4] * x * z;
+            yk[13] = C3[5] * z * (xx - yy);
+            dy[13][0] = 2.0f * C3[5] * x * z; dy[13][1] = -2.0f * C3[5] * y * z; dy[13][2] = C3[5] * (xx - yy);
+            yk[14] = C3[6] * x * (xx - 3.0f * yy);
+            dy[14][0] = C3[6] * (3.0f * xx - 3.0f * yy); dy[14][1] = -6.0f * C3[6] * x * y; dy[14][2] = 0.0f;
+        }
+    }
+}
+
+struct GradOut {
+    float *means, *log_scales, *quats, *logits, *colors, *sh_rest;
+};
+
+template <int DEG>
+__global__ __launch_bounds__(PBWD_THREADS) void k_project_bwd(DevScene sc, const ViewParams *__restrict__ views,
+                                                              uint32_t nviews, const float *__restrict__ g2d,
+                                                              uint64_t g2d_stride, GradOut go) {
+    constexpr int R = (DEG + 1) * (DEG + 1) - 1;
+    const uint32_t i = blockIdx.x * PBWD_THREADS + threadIdx.x;
+    if (i >= sc.n) return;
+    const DevParams &P = sc.prm;
+    const uint64_t i3 = 3ull * i;
+    float p[3], s[3], q[4], col[3], sh[3 * R + 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        p[c] = P.means[i3 + c];
+        s[c] = expf(P.log_scales[i3 + c]);
+        col[c] = P.colors[i3 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q[c] = P.quats[4ull * i + c];
+#pragma unroll
+    for (int c = 0; c < 3 * R; ++c) sh[c] = P.sh_rest[3ull * R * i + c];
+    const float op = sigmoid_f(P.logits[i]);
+    // Rotation of the normalised quaternion (as covariance_f32 builds it) and
+    // the covariance the forward used.
+    float cv[6];
+    float qn[4] = {q[0], q[1], q[2], q[3]};
+    covariance_f32(s, qn, cv);  // normalises qn in place
+    const float qlen = sqrtf((q[0] * q[0] + q[2] * q[2]) + (q[1] * q[1] + q[3] * q[3]));
+    const bool qok = __builtin_isfinite(qlen) && qlen > 1e-12f;
+    const float x = qn[0], y = qn[1], z = qn[2], wq = qn[3];
+    const float Rm[3][3] = {{1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - wq * z), 2.0f * (x * z + wq * y)},
+                            {2.0f * (x * y + wq * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - wq * x)},
+                            {2.0f * (x * z - wq * y), 2.0f * (y * z + wq * x), 1.0f - 2.0f * (x * x + y * y)}};
+    const float Sg[3][3] = {{cv[0], cv[1], cv[2]}, {cv[1], cv[3], cv[4]}, {cv[2], cv[4], cv[5]}};
+    float gp[3] = {0.0f, 0.0f, 0.0f}, gcol[3] = {0.0f, 0.0f, 0.0f}, gsh[3 * R + 1];
+#pragma unroll
+    for (int c = 0; c < 3 * R; ++c) gsh[c] = 0.0f;
+    float Gs[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};  // sum of G + G^T
+    float gop = 0.0f;
+    for (uint32_t vv = 0; vv < nviews; ++vv) {
+        const uint64_t rid = (uint64_t)vv * sc.n + i;
+        float g[NG2D];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) {
+            g[c] = g2d[c * g2d_stride + rid];
+            any = any || g[c] != 0.0f;
+        }
+        if (!any) continue;
+        const ViewCtx &v = views[vv].ctx;
+        const float d0 = p[0] - v.eye[0], d1 = p[1] - v.eye[1], d2 = p[2] - v.eye[2];
+        const float tx = (v.rot[0][0] * d0 + v.rot[0][1] * d1) + v.rot[0][2] * d2;
+        const float ty = (v.rot[1][0] * d0 + v.rot[1][1] * d1) + v.rot[1][2] * d2;
+        const float tz = (v.rot[2][0] * d0 + v.rot[2][1] * d1) + v.rot[2][2] * d2;
+        const float inv_z = 1.0f / tz;
+        const float jxr = tx * inv_z, jyr = ty * inv_z;
+        const bool fx_free = jxr >= -v.lim_x && jxr <= v.lim_x;
+        const bool fy_free = jyr >= -v.lim_y && jyr <= v.lim_y;
+        const float jx = fminf(fmaxf(jxr, -v.lim_x), v.lim_x);
+        const float jy = fminf(fmaxf(jyr, -v.lim_y), v.lim_y);
+        const float f = v.focal * inv_z;
+        float u0[3], u1[3], t0[3], t1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            u0[c] = v.rot[0][c] - jx * v.rot[2][c];
+            u1[c] = v.rot[1][c] - jy * v.rot[2][c];
+            t0[c] = f * u0[c];
+            t1[c] = f * u1[c];
+        }
+        float St0[3], St1[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            St0[r] = Sg[r][0] * t0[0] + Sg[r][1] * t0[1] + Sg[r][2] * t0[2];
+            St1[r] = Sg[r][0] * t1[0] + Sg[r][1] * t1[1] + Sg[r][2] * t1[2];
+        }
+        const float a = (t0[0] * St0[0] + t0[1] * St0[1] + t0[2] * St0[2]) + DILATION;
+        const float b = t0[0] * St1[0] + t0[1] * St1[1] + t0[2] * St1[2];
+        const float cc = (t1[0] * St1[0] + t1[1] * St1[1] + t1[2] * St1[2]) + DILATION;
+        const float det = a * cc - b * b;
+        float inv_det;
+        if (!(det > NEAR_SINGULAR * (a * cc)))
+            inv_det = (float)(1.0 / ((double)a * (double)cc - (double)b * (double)b));
+        else
+            inv_det = 1.0f / det;
+        const float id2 = inv_det * inv_det;
+        const float gA = g[2], gB = g[3], gC = g[4];
+        const float ga = (-cc * cc * gA + b * cc * gB - b * b * gC) * id2;
+        const float gb = (2.0f * b * cc * gA - (a * cc + b * b) * gB + 2.0f * a * b * gC) * id2;
+        const float gc = (-b * b * gA + a * b * gB - a * a * gC) * id2;
+        float gt0[3], gt1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gt0[c] = 2.0f * ga * St0[c] + gb * St1[c];
+            gt1[c] = gb * St0[c] + 2.0f * gc * St1[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                Gs[r][c] += 2.0f * ga * t0[r] * t0[c] + gb * (t0[r] * t1[c] + t1[r] * t0[c]) + 2.0f * gc * t1[r] * t1[c];
+        const float gf = (gt0[0] * u0[0] + gt0[1] * u0[1] + gt0[2] * u0[2]) +
+                         (gt1[0] * u1[0] + gt1[1] * u1[1] + gt1[2] * u1[2]);
+        const float gjx = -f * (gt0[0] * v.rot[2][0] + gt0[1] * v.rot[2][1] + gt0[2] * v.rot[2][2]);
+        const float gjy = -f * (gt1[0] * v.rot[2][0] + gt1[1] * v.rot[2][1] + gt1[2] * v.rot[2][2]);
+        const float iz2 = inv_z * inv_z;
+        float gtx = g[0] * v.focal * inv_z;
+        float gty = g[1] * v.focal * inv_z;
+        float gtz = -(g[0] * v.focal * tx + g[1] * v.focal * ty) * iz2 - gf * v.focal * iz2;
+        if (fx_free) {
+            gtx += gjx * inv_z;
+            gtz -= gjx * tx * iz2;
+        }
+        if (fy_free) {
+            gty += gjy * inv_z;
+            gtz -= gjy * ty * iz2;
+        }
+        float gd[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gd[c] = v.rot[0][c] * gtx + v.rot[1][c] * gty + v.rot[2][c] * gtz;
+        // Colour: max(base + SH(dir), 0) with dir = normalize(p - eye).
+        float pre[3] = {col[0], col[1], col[2]};
+        float ux = 0.0f, uy = 0.0f, uz = 0.0f, rcp = 0.0f;
+        if constexpr (DEG > 0) {
+            const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+            rcp = 1.0f / len;
+            if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+                ux = d0 * rcp;
+                uy = d1 * rcp;
+                uz = d2 * rcp;
+            } else {
+                rcp = 0.0f;
+            }
+            sh_color<DEG>(sh, ux, uy, uz, pre);
+        }
+        float gcv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gcv[c] = pre[c] > 0.0f ? g[6 + c] : 0.0f;
+            gcol[c] += gcv[c];
+        }
+        if constexpr (DEG > 0) {
+            float yk[R > 0 ? R : 1], dyk[R > 0 ? R : 1][3];
+            sh_basis<DEG>(ux, uy, uz, yk, dyk);
+            float gdir[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const float wk = gcv[0] * sh[3 * k] + gcv[1] * sh[3 * k + 1] + gcv[2] * sh[3 * k + 2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    gsh[3 * k + c] += gcv[c] * yk[k];
+                    gdir[c] += wk * dyk[k][c];
+                }
+            }
+            const float dg = ux * gdir[0] + uy * gdir[1] + uz * gdir[2];
+            gd[0] += (gdir[0] - ux * dg) * rcp;
+            gd[1] += (gdir[1] - uy * dg) * rcp;
+            gd[2] += (gdir[2] - uz * dg) * rcp;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gp[c] += gd[c];
+        gop += g[5];
+    }
+    // Sigma = M M^T with M = R diag(s): dL/dM = (G + G^T) M.
+    float gs[3], gR[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float gM = (Gs[r][0] * Rm[0][c] + Gs[r][1] * Rm[1][c] + Gs[r][2] * Rm[2][c]) * s[c];
+            acc += gM * Rm[r][c];
+            gR[r][c] = gM * s[c];
+        }
+        gs[c] = acc;
+    }
+    // d R / d (x, y, z, w) of the normalised quaternion.
+    float gqn[4];
+    gqn[0] = 2.0f * (y * (gR[0][1] + gR[1][0]) + z * (gR[0][2] + gR[2][0]) + wq * (gR[2][1] - gR[1][2])) -
+             4.0f * x * (gR[1][1] + gR[2][2]);
+    gqn[1] = 2.0f * (x * (gR[0][1] + gR[1][0]) + z * (gR[1][2] + gR[2][1]) + wq * (gR[0][2] - gR[2][0])) -
+             4.0f * y * (gR[0][0] + gR[2][2]);
+    gqn[2] = 2.0f * (x * (gR[0][2] + gR[2][0]) + y * (gR[1][2] + gR[2][1]) + wq * (gR[1][0] - gR[0][1])) -
+             4.0f * z * (gR[0][0] + gR[1][1]);
+    gqn[3] = 2.0f * (x * (gR[2][1] - gR[1][2]) + y * (gR[0][2] - gR[2][0]) + z * (gR[1][0] - gR[0][1]));
+    const float qd = qn[0] * gqn[0] + qn[1] * gqn[1] + qn[2] * gqn[2] + qn[3] * gqn[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) go.quats[4ull * i + c] = qok ? (gqn[c] - qn[c] * qd) / qlen : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        go.means[i3 + c] = gp[c];
+        go.log_scales[i3 + c] = gs[c] * s[c];
+        go.colors[i3 + c] = gcol[c];
+    }
+    go.logits[i] = gop * op * (1.0f - op);
+#pragma unroll
+    for (int c = 0; c < 3 * R; ++c) go.sh_rest[3ull * R * i + c] = gsh[c];
+}
+
+template <int DEG>
+void launch_pbwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, const float *g2d,
+                 uint64_t g2d_stride, const GradOut &go) {
+    const uint32_t grid = (sc.n + PBWD_THREADS - 1) / PBWD_THREADS;
+    hipLaunchKernelGGL(k_project_bwd<DEG>, dim3(grid), dim3(PBWD_THREADS), 0, s, sc, views, nviews, g2d, g2d_stride,
+                       go);
+}
+
+}  // namespace
+
+hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
+                            const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
+                            uint64_t gstride) {
+    if (total_tiles == 0) return hipSuccess;
+    const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
+    hipLaunchKernelGGL(k_blend_bwd, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, views, nviews, total_tiles,
+                       tile_start, sval, powner, rec, pix_T, pix_last, gpair, gstride, per_xcd);
+    return hipGetLastError();
+}
+
+uint64_t seg_blocks(uint64_t np) { return (np + SR_THREADS - 1) / SR_THREADS; }
+
+hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *powner, uint64_t np,
+                             float *g2d, uint64_t g2d_stride, float *part) {
+    if (np == 0) return hipSuccess;
+    const uint64_t nb = seg_blocks(np);
+    hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(SR_THREADS), 0, s, gpair, gstride, powner, np, g2d,
+                       g2d_stride, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_seg_carry, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, powner, np, (uint32_t)nb,
+                       part, g2d, g2d_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
+                              const float *g2d, uint64_t g2d_stride, float *g_means, float *g_log_scales,
+                              float *g_quats, float *g_logits, float *g_colors, float *g_sh_rest) {
+    if (sc.n == 0) return hipSuccess;
+    const GradOut go{g_means, g_log_scales, g_quats, g_logits, g_colors, g_sh_rest};
+    switch (sc.deg) {
+    case 0: launch_pbwd<0>(s, sc, views, nviews, g2d, g2d_stride, go); break;
+    case 1: launch_pbwd<1>(s, sc, views, nviews, g2d, g2d_stride, go); break;
+    case 2: launch_pbwd<2>(s, sc, views, nviews, g2d, g2d_stride, go); break;
+    default: launch_pbwd<3>(s, sc, views, nviews, g2d, g2d_stride, go); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sp

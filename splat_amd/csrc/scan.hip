@@ -75,6 +75,11 @@ __global__ __launch_bounds__(256) void k_fill(uint32_t *__restrict__ p, uint64_t
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
 }
 
+__global__ __launch_bounds__(256) void k_fill_f32(float *__restrict__ p, uint64_t n, float value) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+}
+
 }  // namespace
 
 static_assert(RADIX_BINS == SCAN_THREADS, "digit-row bases use one thread per digit");
@@ -98,6 +103,13 @@ hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(hipStream_t s, float *p, uint64_t n, float value) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid), dim3(256), 0, s, p, n, value);
     return hipGetLastError();
 }
 

@@ -45,11 +45,15 @@ struct ViewCtx {
 
 // One view of a batch, in device memory: its context, the first global tile id
 // of its tiles (views' tiles are numbered consecutively) and its RGB8 output.
+// The differentiable path writes a float image (fimg, w*h*3, C + T*bg before
+// quantisation) instead and reads the loss gradient of that image (gimg).
 struct ViewParams {
     ViewCtx ctx;
     uint32_t tile_base;
     uint32_t pad0;
     uint8_t *out;
+    float *fimg;
+    const float *gimg;
 };
 
 // Device-resident scene in structure-of-arrays form: array `a` of n floats
@@ -62,6 +66,16 @@ struct ViewParams {
 // arrays in soa16: 0-2 scale, 3-6 quaternion x,y,z,w, 7-9 base colour,
 // 10.. SH rest (coefficient c channel ch at 10 + 3c + ch). The covariance is
 // rebuilt in f32 on the device.
+//
+// The differentiable path (storage STORE_PARAMS) reads raw, unactivated
+// parameters instead, row-major per gaussian as the caller's tensors hold
+// them: means [n][3], log-scales [n][3], quaternions [n][4] (x,y,z,w,
+// normalised on the fly), opacity logits [n], base colours [n][3], SH rest
+// [n][rest][3]. Activation (exp, sigmoid, normalisation, covariance, k) is
+// computed per use.
+struct DevParams {
+    const float *means, *log_scales, *quats, *logits, *colors, *sh_rest;
+};
 struct DevScene {
     const float *soa;
     const uint16_t *soa16;
@@ -69,8 +83,17 @@ struct DevScene {
     uint32_t n;
     uint32_t deg;
     uint32_t rest;
-    uint32_t storage;  // SP_STORE_*
+    uint32_t storage;  // SP_STORE_* or STORE_PARAMS
+    DevParams prm;
 };
+constexpr int STORE_PARAMS = 3;
+// Per-pair gradient of the loss with respect to a projected record, reduced
+// over the pixels of the pair's tile: mean x, y; conic a, b, c; opacity;
+// red, green, blue.
+constexpr int NG2D = 9;
+// Gradient floats per gaussian: mean 3, log-scale 3, quaternion 4, opacity
+// logit 1, base colour 3, SH rest 3 * rest.
+constexpr int NG3D_BASE = 14;
 constexpr int A_POS = 0, A_COV = 3, A_OPAC = 9, A_K = 10, A_COL = 11, A_SH = 14;
 constexpr int L_POS = 0, L_OPAC = 3, L_K = 4;                            // f32 arrays, 16-bit storage
 constexpr int H_SCALE = 0, H_QUAT = 3, H_COL = 7, H_SH = 10;             // 16-bit arrays
@@ -83,7 +106,7 @@ constexpr float NEAR_SINGULAR = 1e-3f;
 // v * n + i. Layout:
 //   r0 = x, y, -0.5*conic_a, -conic_b
 //   r1 = -0.5*conic_c, opacity, red, green
-//   r2 = blue, bits(x0 | y0 << 16), bits(x1 | y1 << 16), depth
+//   r2 = blue, bits(x0 | y0 << 16), bits(x1 | y1 << 16), falloff threshold -k^2/2 - 0.02
 // The pre-scaled conic terms are the exact factors raster_tile multiplies by
 // (src/render/cpu.rs:662, :671-672), so precomputing them changes no bits.
 

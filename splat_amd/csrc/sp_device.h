@@ -49,6 +49,26 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     return pre + inc - v;
 }
 
+// The view of a batch that owns global tile gt: the last view whose
+// tile_base <= gt (bases ascend with the view index).
+__device__ __forceinline__ uint32_t view_of_tile(const ViewParams *views, uint32_t nviews, uint32_t gt) {
+    uint32_t lo = 0, hi = nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].tile_base <= gt)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 // Rust `as u32` on f32 (saturating, NaN -> 0).
 __device__ __forceinline__ uint32_t sat_u32(float v) {
     if (!(v > 0.0f)) return 0u;

@@ -50,10 +50,12 @@ uint32_t emit_blocks(uint32_t n);
 // One (global tile id, record index) pair per overlapped 16x16 tile of each
 // view's depth-ordered splats, views in order. bcnt: nviews rows of
 // emit_blocks(n) block counts; vtot: nviews entries. Sets counters[CNT_PAIRS]
-// (exact) and counters[CNT_OVERFLOW] when above pair_cap.
+// (exact) and counters[CNT_OVERFLOW] when above pair_cap. With powner set,
+// pval[e] = e and powner[e] = record index instead.
 hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, uint32_t *bcnt,
-                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap, uint32_t *counters);
+                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
+                       uint32_t *counters);
 // tile_start[0..num_tiles] from the tile-sorted pair keys.
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *counters, uint64_t pair_cap,
                          uint32_t num_tiles, uint32_t *tile_start);
@@ -62,7 +64,36 @@ hipError_t launch_view_pairs(hipStream_t s, const ViewParams *views, uint32_t nv
                              const uint32_t *tile_start, uint32_t *view_pairs);
 
 // ---- blend.hip: (e) front-to-back blend of every tile of every view ----
+// batch = records staged through LDS per step (64, 128 or 256).
 hipError_t launch_blend(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters);
+                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters,
+                        int batch);
+// Differentiable forward: sval holds emission indices (record = powner[e]);
+// writes each view's float image (ViewParams::fimg) and the per-pixel final
+// transmittance / last contributing list position (total_tiles * 256 each).
+hipError_t launch_blend_diff(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                             const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
+                             const uint4 *rec, uint32_t *counters, float *pix_T, uint32_t *pix_last, int batch);
+
+// ---- diff.hip: adjoint of the blend and of the projection ----
+// Per-pair gradients (NG2D arrays of gstride floats, indexed by emission
+// index) of every tile's list up to its last used position; pairs after it
+// are left untouched (the caller zero-fills gpair).
+hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
+                            const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
+                            uint64_t gstride);
+// Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.
+uint64_t seg_blocks(uint64_t np);
+// g2d[c * g2d_stride + record] = sum of gpair over the record's pairs (records
+// without pairs are untouched; the caller zero-fills g2d).
+hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *powner, uint64_t np,
+                             float *g2d, uint64_t g2d_stride, float *part);
+// Per-gaussian parameter gradients (sc.storage == STORE_PARAMS), summed over views.
+hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
+                              const float *g2d, uint64_t g2d_stride, float *g_means, float *g_log_scales,
+                              float *g_quats, float *g_logits, float *g_colors, float *g_sh_rest);
+// p[0..n) = value for float arrays.
+hipError_t launch_fill_f32(hipStream_t s, float *p, uint64_t n, float value);
 
 }  // namespace sp

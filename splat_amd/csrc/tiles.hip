@@ -12,7 +12,10 @@
 // a splat covering 1000 tiles costs no more than 1000 coalesced writes
 // spread over the block. Pairs come out view by view in (depth, scene index)
 // order, carrying a global tile id (view tile base + tile) and a record index
-// (view * n + scene index), ready for the stable tile-id sort.
+// (view * n + scene index), ready for the stable tile-id sort. The
+// differentiable path sorts the emission index instead and keeps the record
+// index per emission index (powner), so that per-pair gradients can be
+// written back in emission order, where each splat's pairs are contiguous.
 //
 // Ranges: bucket bounds tile_start[t] (src/render/cpu.rs:150-163) from the
 // boundaries of the tile-sorted keys; empty tiles between two boundaries get
@@ -71,7 +74,8 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand(const uint32_t *__
                                                               const uint32_t *__restrict__ boff,
                                                               const uint32_t *__restrict__ vbase_pairs,
                                                               uint32_t *__restrict__ pkey, uint32_t *__restrict__ pval,
-                                                              uint64_t pair_cap, uint32_t *__restrict__ counters) {
+                                                              uint32_t *__restrict__ powner, uint64_t pair_cap,
+                                                              uint32_t *__restrict__ counters) {
     __shared__ uint32_t s_pre[EMIT_THREADS];
     __shared__ uint32_t s_org[EMIT_THREADS];  // first global tile id of the rectangle
     __shared__ uint32_t s_rw[EMIT_THREADS];   // rectangle width in tiles
@@ -110,7 +114,14 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand(const uint32_t *__
         const uint64_t pos = base + q;
         if (pos < pair_cap) {
             pkey[pos] = s_org[lo] + ry * tiles_x + rx;
-            pval[pos] = s_id[lo];
+            if (powner) {
+                // Differentiable path: the pair carries its emission index; the
+                // record index stays behind in emission order.
+                pval[pos] = (uint32_t)pos;
+                powner[pos] = s_id[lo];
+            } else {
+                pval[pos] = s_id[lo];
+            }
         }
     }
 }
@@ -154,7 +165,8 @@ uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THRE
 
 hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, uint32_t *bcnt,
-                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap, uint32_t *counters) {
+                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
+                       uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     hipLaunchKernelGGL(k_emit_count, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv, vis_count,
                        rec, views, bcnt);
@@ -164,7 +176,7 @@ hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint32_t n_splats, u
     if (e == hipSuccess) e = launch_scan_rows(s, vtot, nviews, 1, nullptr, counters + CNT_PAIRS);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_emit_expand, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv, vis_count,
-                       rec, views, bcnt, vtot, pkey, pval, pair_cap, counters);
+                       rec, views, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
     return hipGetLastError();
 }
 
