@@ -13,6 +13,12 @@ Also reported beside the headline value:
     one sp_render at a time,
   * cpu_baseline: the oracle (C++ restatement of splat-tui's `cpu` backend) on a
     bounded sample of the same views, timed on this host's cores.
+  * precision: 3,000,000 splats stored as f32 / FP16 / BF16: views/s and each
+    stage's error against the FP64 oracle (conics, colours, final image).
+  * differentiation: 500,000 splats, 8 views at 1280x720, L2 loss against
+    targets rendered from parameters perturbed by N(0, 0.01): forward and
+    backward wall times, and the gradient checked against the FP64 reference
+    gradient and FP64 central differences on 256 random parameters.
 
 python bench.py [--gpus 1] [--steps 40] [--warmup 5]
 """
@@ -54,6 +60,11 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-splats", type=int, default=3_000_000)
+    ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--diff-splats", type=int, default=500_000)
+    ap.add_argument("--gradcheck-params", type=int, default=256)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -167,7 +178,169 @@ def main() -> None:
         except Exception as e:
             result["cpu_baseline"] = {"error": repr(e)}
     r.close()
+    del r, scene
+    if not args.no_diff:
+        try:
+            result["differentiation"] = bench_diff(S, torch, args)
+        except Exception as e:
+            result["differentiation"] = {"error": repr(e)}
+    if not args.no_precision:
+        try:
+            result["precision"] = bench_precision(S, torch, args, W, H)
+        except Exception as e:
+            result["precision"] = {"error": repr(e)}
     print(json.dumps(result))
+
+
+def host_cores() -> int:
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return min(cores, 16)
+
+
+def bench_diff(S, torch, args) -> dict:
+    """Forward + backward of an L2 image loss for 500k SH-3 splats over 8 views at
+    1280x720 (wall times around synchronous calls, median of 5), then the GPU
+    gradient against the FP64 reference: relative L2 error per parameter group and
+    central differences (step 1e-6) on random parameters of visible splats."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from splat_amd.diff import PARAM_KEYS, DiffRenderer, params_from_scene
+
+    n = args.diff_splats
+    log(f"differentiation: {n} splats, 8 views 1280x720")
+    scene = S.random_scene(n, seed=1, sh_degree=3)
+    prm = params_from_scene(scene)
+    views = S.orbit_views(scene, 40, 20.0)[::5]
+    sizes = [(1280, 720)] * len(views)
+    bg = (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(0)
+    tprm = {k: (v + rng.normal(0, 0.01, v.shape)).astype(np.float32) for k, v in prm.items()}
+    dev = lambda d: {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in d.items()}
+    P, TP = dev(prm), dev(tprm)
+    r = DiffRenderer(n, 3)
+    targets = [t.clone() for t in r.forward(TP, views, sizes, bg)]
+    fwd, bwd = [], []
+    for it in range(6):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        imgs = r.forward(P, views, sizes, bg)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gimg = [2.0 * (imgs[v] - targets[v]) for v in range(len(views))]
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g = r.backward(P, gimg)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if it:  # first iteration warms allocations
+            fwd.append((t1 - t0) * 1e3)
+            bwd.append((t3 - t2) * 1e3)
+    loss = float(sum(((imgs[v] - targets[v]) ** 2).sum().item() for v in range(len(views))))
+    pairs = int(sum(st.overlaps for st in r.last_stats))
+    out = {
+        "workload": f"{n} random SH-3 splats, {len(views)} orbit views at 1280x720, L2 loss vs targets from "
+                    "parameters + N(0, 0.01)",
+        "forward_ms": round(float(np.median(fwd)), 3),
+        "backward_ms": round(float(np.median(bwd)), 3),
+        "backward_over_forward": round(float(np.median(bwd)) / float(np.median(fwd)), 3),
+        "pairs": pairs,
+        "loss": loss,
+    }
+    if args.gradcheck_params <= 0:
+        return out
+    from oracle import oracle as O
+
+    cores = host_cores()
+    log(f"fp64 reference gradient on {cores} threads")
+    gn = {k: g[k].cpu().numpy() for k in PARAM_KEYS}
+    G = np.concatenate([gn["means"], gn["log_scales"], gn["quats"], gn["opacity_logits"].reshape(n, 1),
+                        gn["colors"], gn["sh_rest"].reshape(n, -1)], axis=1).astype(np.float64)
+    tnp = [t.cpu().numpy() for t in targets]
+    t0 = time.perf_counter()
+    D = O.DiffF64(prm, 3, views, sizes, bg, tnp, threads=cores)
+    G64 = D.grad_flat()
+    groups = {"means": slice(0, 3), "log_scales": slice(3, 6), "quats": slice(6, 10),
+              "opacity_logits": slice(10, 11), "colors": slice(11, 14), "sh_rest": slice(14, None)}
+    rel_l2 = {k: float(np.linalg.norm(G[:, sl] - G64[:, sl]) / max(np.linalg.norm(G64[:, sl]), 1e-30))
+              for k, sl in groups.items()}
+    rows = np.nonzero(np.abs(G64).sum(1))[0]
+    prng = np.random.default_rng(1)
+    picks = [(int(prng.choice(rows)), int(prng.integers(0, D.np))) for _ in range(args.gradcheck_params)]
+    # Central differences at two steps. The loss is only piecewise smooth (alpha
+    # crosses 1/255, T crosses 1e-4, boxes move by a pixel); where the two steps
+    # disagree, the stencil straddles such a jump and no difference quotient
+    # estimates the derivative. Errors are relative with a 1e-6 floor (the
+    # difference quotient's round-off level for this loss).
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        fd5 = list(ex.map(lambda ik: D.fd(ik[0], ik[1], 1e-5), picks))
+        fd6 = list(ex.map(lambda ik: D.fd(ik[0], ik[1], 1e-6), picks))
+    smooth = np.array([abs(a - b) <= 1e-3 * max(abs(a), abs(b)) + 1e-7 for a, b in zip(fd5, fd6)])
+    rel = np.array([abs(G[i, k] - fd) / max(abs(fd), abs(G[i, k]), 1e-6) for (i, k), fd in zip(picks, fd5)])
+    rs = rel[smooth] if smooth.any() else np.zeros(1)
+    out["gradient_check"] = {
+        "fp64_loss": D.loss(),
+        "rel_l2_vs_fp64_gradient": {k: float(f"{v:.3e}") for k, v in rel_l2.items()},
+        "fd_params": len(picks),
+        "fd_step": 1e-5,
+        "fd_rel_err_median": float(f"{np.median(rel):.3e}"),
+        "fd_rel_err_p95": float(f"{np.percentile(rel, 95):.3e}"),
+        "fd_rel_err_max": float(f"{rel.max():.3e}"),
+        "fd_smooth_params": int(smooth.sum()),
+        "fd_smooth_rel_err_median": float(f"{np.median(rs):.3e}"),
+        "fd_smooth_rel_err_max": float(f"{rs.max():.3e}"),
+        "fp64_seconds": round(time.perf_counter() - t0, 1),
+        "cores": cores,
+    }
+    return out
+
+
+def bench_precision(S, torch, args, W, H) -> dict:
+    """3M SH-3 splats with the covariance factors (scale, rotation), colours and SH
+    stored as f32 / FP16 / BF16: views/s over 20 orbit views, and for one view the
+    error of each stage against the FP64 oracle (projection of every splat, full
+    FP64 render)."""
+    from oracle import oracle as O
+
+    n = args.precision_splats
+    log(f"precision: {n} splats")
+    scene = S.random_scene(n, seed=1, sh_degree=3)
+    views = S.orbit_views(scene, 40, 20.0)
+    v0 = views[0]
+    cores = host_cores()
+    v64, p64 = O.project_f64(scene, v0, W, H, threads=cores)
+    rgb64, img64 = O.render_f64(scene, v0, W, H, (0, 0, 0), threads=cores)
+    out = {"workload": f"{n} random SH-3 splats, {W}x{H}, 40-frame orbit", "storages": {}}
+    dev = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
+    for storage in ("f32", "f16", "bf16"):
+        r = S.HipRenderer(scene, storage=storage)
+        for i in range(3):
+            r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k = 20
+        for i in range(k):
+            r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
+        torch.cuda.synchronize()
+        vps = k / (time.perf_counter() - t0)
+        valid, p, _ = r.debug_project(v0, W, H)
+        both = valid & v64
+        conic = p[both][:, 2:5].astype(np.float64)
+        c64 = p64[both][:, 2:5]
+        crel = np.abs(conic - c64) / np.abs(c64).max(axis=1, keepdims=True)
+        col = np.abs(p[both][:, 5:8].astype(np.float64) - p64[both][:, 5:8])
+        img, _ = r.render(v0, W, H, (0, 0, 0))
+        d = np.abs(img.pixels.astype(int) - img64.astype(int))
+        out["storages"][storage] = {
+            "views_per_s": round(vps, 3),
+            "device_bytes": r.frame_info()["device_bytes"],
+            "cull_mismatch_vs_fp64": int((valid != v64).sum()),
+            "conic_rel_err_max": float(f"{crel.max():.3e}"), "conic_rel_err_mean": float(f"{crel.mean():.3e}"),
+            "color_abs_err_max": float(f"{col.max():.3e}"), "color_abs_err_mean": float(f"{col.mean():.3e}"),
+            "image_psnr_vs_fp64": round(float(S.psnr(img.pixels, img64)), 3),
+            "image_max_abs_err_vs_fp64": int(d.max()),
+        }
+        r.close()
+    return out
 
 
 def bench_batched(S, torch, scene, nviews: int) -> dict:
@@ -212,8 +385,7 @@ def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:
     view byte-for-byte against the HIP image."""
     from oracle import oracle as O
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    cores = min(cores, 16)
+    cores = host_cores()
     cpu = O.CpuTiled(scene, threads=cores)
     log(f"cpu baseline on {cores} threads")
     img, _ = cpu.render(views[0], W, H, (0, 0, 0))  # warm

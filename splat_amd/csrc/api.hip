@@ -98,10 +98,12 @@ double now_ms() {
 // (device-resident) view table.
 struct GraphKey {
     uint32_t nviews = 0, total_tiles = 0, tile_bits = 0, with_blend = 0;
+    uint32_t split_shift = 0, split_kmin = 0, blend_batch = 0;
     uint64_t generation = 0;
     bool operator==(const GraphKey &o) const {
         return nviews == o.nviews && total_tiles == o.total_tiles && tile_bits == o.tile_bits &&
-               with_blend == o.with_blend && generation == o.generation;
+               with_blend == o.with_blend && split_shift == o.split_shift && split_kmin == o.split_kmin &&
+               blend_batch == o.blend_batch && generation == o.generation;
     }
 };
 
@@ -124,6 +126,7 @@ struct sp_handle {
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
     uint32_t *d_vstat = nullptr;  // 3 x view_cap: visible, pairs, emission view bases
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
+    uint32_t *d_bcnt_all = nullptr;  // view_cap x block counts of the unsplit emission
     ViewParams *d_views = nullptr, *h_views = nullptr;
     uint32_t *d_hist = nullptr;  // view_cap x 4 passes x 256 digits
     uint32_t *d_table = nullptr;  // radix per-block digit counts
@@ -137,11 +140,18 @@ struct sp_handle {
     uint64_t rgb_cap = 0;
     uint32_t *d_counters = nullptr;
     uint32_t *h_counters = nullptr;  // pinned: CNT_SLOTS counters then 2*view_cap view stats
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     sp_frame_info last{};
     int pair_buf = 0;
     int depth_buf = 0;  // ping-pong buffer holding the last depth sort's output
     int blend_batch = 256;  // records staged through LDS per blend step (64, 128 or 256)
+    // Front-to-back split of the plain path: the first phase emits each view's
+    // nearest max(split_kmin, vis >> split_shift) splats (split_shift 0: no split).
+    uint32_t split_shift = 3, split_kmin = 16384;
+    uint32_t *d_done = nullptr;  // split_cap tiles: finished by the first phase
+    uint32_t *d_ip = nullptr;    // split_cap tiles: row prefix counts of unfinished tiles
+    float *d_pstate = nullptr;   // split_cap * 256 pixels: (T, r, g, b) between the phases
+    uint64_t split_cap = 0;
     // Differentiable path (sp_diff_create): raw parameters on the caller's
     // device memory, per-pair owners, per-pixel blend state, gradient scratch.
     bool diff = false;
@@ -254,6 +264,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     }
     dfree(h, h->d_vstat, old * 12);
     dfree(h, h->d_bcnt, old * blocks * 4);
+    dfree(h, h->d_bcnt_all, old * blocks * 4);
     dfree(h, h->d_views, old * sizeof(ViewParams));
     dfree(h, h->d_hist, old * 4 * sp::RADIX_BINS * 4);
     if (h->h_views) (void)hipHostFree(h->h_views);
@@ -269,6 +280,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     }
     if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 12);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt, v * blocks * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_bcnt_all, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_views, v * sizeof(ViewParams));
     if (!rc) rc = dalloc(h, (void **)&h->d_hist, v * 4 * sp::RADIX_BINS * 4);
     if (rc) return rc;
@@ -300,6 +312,21 @@ int ensure_pix(sp_handle *h, uint64_t tiles) {
     if (!rc) rc = dalloc(h, (void **)&h->d_pixlast, need * 4);
     if (rc) return rc;
     h->pix_cap = need;
+    return SP_OK;
+}
+
+// Tile flags, row prefixes and pixel state of the front-to-back split.
+int ensure_split(sp_handle *h, uint64_t tiles) {
+    if (tiles <= h->split_cap) return SP_OK;
+    dfree(h, h->d_done, h->split_cap * 4);
+    dfree(h, h->d_ip, h->split_cap * 4);
+    dfree(h, h->d_pstate, h->split_cap * sp::BLEND_THREADS * 16);
+    h->split_cap = 0;
+    int rc = dalloc(h, (void **)&h->d_done, tiles * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_ip, tiles * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_pstate, tiles * sp::BLEND_THREADS * 16);
+    if (rc) return rc;
+    h->split_cap = tiles;
     return SP_OK;
 }
 
@@ -367,7 +394,10 @@ int setup(sp_handle *h) {
 }
 
 // Enqueue the whole pipeline for the first `nviews` entries of h_views. The
-// sorted pairs end in ping-pong buffer *pair_buf.
+// sorted pairs of the (last) tile sort end in ping-pong buffer *pair_buf.
+// BL_RGB with a split runs the emission, tile sort, ranges and blend twice:
+// the first phase for each view's nearest splats over all tiles, the second
+// for the rest over the tiles the first blend left unfinished.
 int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, int mode, bool events,
             int *pair_buf) {
     const sp::DevScene sc = dev_scene(h);
@@ -375,32 +405,54 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     uint32_t *vis = h->d_vstat;
     uint32_t *view_pairs = h->d_vstat + h->view_cap;
     uint32_t *vtot = h->d_vstat + 2 * h->view_cap;
+    const bool split = mode == BL_RGB && h->split_shift > 0;
+    const sp::EmitSplit es = split ? sp::EmitSplit{h->split_kmin, h->split_shift} : sp::EmitSplit{0u, 0u};
+    uint32_t *powner = mode == BL_DIFF ? h->d_powner : nullptr;
+    auto mark = [&](int k) { return events ? hipEventRecord(h->ev[k], s) : hipSuccess; };
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
-    if (events) HIP_TRY(hipEventRecord(h->ev[0], s));
+    HIP_TRY(mark(0));
     // The uncompacted depth keys live in the depth sort's second buffer until the sort needs it.
     HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_dkey[1], h->d_bcnt, vis, h->d_dkey[0],
                                h->d_dval[0], nullptr, nullptr, nullptr));
     int di = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_hist, h->d_table, &di));
     h->depth_buf = di;
-    HIP_TRY(sp::launch_emit(s, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, h->d_bcnt, vtot,
-                            h->d_pkey[0], h->d_pval[0], mode == BL_DIFF ? h->d_powner : nullptr, h->pair_cap,
-                            h->d_counters));
-    if (events) HIP_TRY(hipEventRecord(h->ev[1], s));
+    HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, es, nullptr,
+                            h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0], powner,
+                            h->pair_cap, h->d_counters));
+    HIP_TRY(mark(1));
     int pi = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
                                   tile_bits, h->d_hist, h->d_table, &pi));
-    HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters, h->pair_cap, total_tiles, h->d_tile_start));
-    HIP_TRY(sp::launch_view_pairs(s, h->d_views, nviews, total_tiles, h->d_tile_start, view_pairs));
-    if (events) HIP_TRY(hipEventRecord(h->ev[2], s));
-    if (mode == BL_RGB)
-        HIP_TRY(sp::launch_blend(s, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi], h->d_rec,
-                                 h->d_counters, h->blend_batch));
-    if (mode == BL_DIFF)
-        HIP_TRY(sp::launch_blend_diff(s, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
-                                      h->d_powner, h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast,
-                                      h->blend_batch));
-    if (events) HIP_TRY(hipEventRecord(h->ev[3], s));
+    HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS, h->pair_cap, total_tiles,
+                              h->d_tile_start));
+    HIP_TRY(mark(2));
+    if (mode != BL_NONE) {
+        const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
+        HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi], powner,
+                                 h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast, h->d_done, h->d_pstate,
+                                 h->blend_batch));
+    }
+    HIP_TRY(mark(3));
+    if (split) {
+        HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip));
+        HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, es,
+                                h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0],
+                                nullptr, h->pair_cap, h->d_counters));
+    }
+    HIP_TRY(mark(4));
+    if (split) {
+        HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS2, 1, 0, h->pair_cap,
+                                      tile_bits, h->d_hist, h->d_table, &pi));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS2, h->pair_cap, total_tiles,
+                                  h->d_tile_start));
+    }
+    HIP_TRY(mark(5));
+    if (split)
+        HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
+                                 nullptr, h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
+                                 h->blend_batch));
+    HIP_TRY(mark(6));
     HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, sp::CNT_SLOTS * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS, vis, nviews * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS + nviews, view_pairs, nviews * 4, hipMemcpyDeviceToHost, s));
@@ -435,6 +487,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     const uint32_t total_tiles = (uint32_t)total;
     rc = ensure_tiles(h, total_tiles);
     if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
+    if (!rc && mode == BL_RGB && h->split_shift > 0) rc = ensure_split(h, total_tiles);
     if (rc) return rc;
     const int tile_bits = (int)bits_for(total_tiles ? total_tiles - 1 : 0);
     const bool events = h->opt.timing && !use_graph;
@@ -446,6 +499,9 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             key.total_tiles = total_tiles;
             key.tile_bits = (uint32_t)tile_bits;
             key.with_blend = (uint32_t)mode;
+            key.split_shift = h->split_shift;
+            key.split_kmin = h->split_kmin;
+            key.blend_batch = (uint32_t)h->blend_batch;
             key.generation = h->generation;
             if (!h->gexec || !(key == h->gkey)) {
                 drop_graph(h);
@@ -471,7 +527,8 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         HIP_TRY(hipStreamSynchronize(h->stream));
         h->pair_buf = pb;
         if (h->h_counters[sp::CNT_OVERFLOW] && attempt < 3) {
-            const uint64_t p = h->h_counters[sp::CNT_PAIRS];
+            uint64_t p = h->h_counters[sp::CNT_PAIRS];
+            if (mode == BL_RGB && h->split_shift > 0) p = std::max<uint64_t>(p, h->h_counters[sp::CNT_PAIRS2]);
             rc = ensure_pairs(h, p + p / 4 + 1024);
             if (rc) return rc;
             continue;
@@ -481,9 +538,11 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     }
     float tp = 0, ts = 0, tr = 0;
     if (events) {
-        HIP_TRY(hipEventElapsedTime(&tp, h->ev[0], h->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&ts, h->ev[1], h->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&tr, h->ev[2], h->ev[3]));
+        float e[6];
+        for (int k = 0; k < 6; ++k) HIP_TRY(hipEventElapsedTime(&e[k], h->ev[k], h->ev[k + 1]));
+        tp = e[0] + e[3];  // projection, depth sort, both emission phases
+        ts = e[1] + e[4];  // both tile sorts and ranges
+        tr = e[2] + e[5];  // both blends
     }
     const double total_ms = now_ms() - t0;
     const uint32_t *vis = h->h_counters + sp::CNT_SLOTS;
@@ -506,7 +565,9 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     h->last.height = wh[1];
     h->last.tiles = total_tiles;
     h->last.visible = vis_sum;
-    h->last.pairs = h->h_counters[sp::CNT_PAIRS];
+    uint64_t pair_sum = 0;
+    for (uint32_t v = 0; v < nviews; ++v) pair_sum += vpairs[v];
+    h->last.pairs = pair_sum;
     h->last.pairs_staged = blend ? h->h_counters[sp::CNT_STAGED] : 0;
     h->last.pair_capacity = h->pair_cap;
     h->last.device_bytes = h->bytes;
@@ -591,8 +652,9 @@ int finish_handle(sp_handle *h) {
         hipStreamSynchronize(h->stream) != hipSuccess)
         return fail(SP_E_DEVICE, "counter init failed");
     rc = ensure_views(h, 1);
-    const uint64_t cap0 =
-        h->opt.initial_pairs ? h->opt.initial_pairs : std::max<uint64_t>(16ull * h->n, 1ull << 20);
+    // The split plain path materialises only the first phase's pairs at once.
+    const uint64_t cap0 = h->opt.initial_pairs ? h->opt.initial_pairs
+                                               : std::max<uint64_t>((h->diff ? 16ull : 4ull) * h->n, 1ull << 20);
     if (!rc) rc = ensure_pairs(h, cap0);
     return rc;
 }
@@ -635,6 +697,7 @@ void sp_destroy(sp_handle *h) {
     }
     (void)hipFree(h->d_vstat);
     (void)hipFree(h->d_bcnt);
+    (void)hipFree(h->d_bcnt_all);
     (void)hipFree(h->d_views);
     (void)hipFree(h->d_table);
     (void)hipFree(h->d_tile_start);
@@ -647,6 +710,9 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_gpair);
     (void)hipFree(h->d_g2d);
     (void)hipFree(h->d_part);
+    (void)hipFree(h->d_done);
+    (void)hipFree(h->d_ip);
+    (void)hipFree(h->d_pstate);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->h_views) (void)hipHostFree(h->h_views);
     for (auto &e : h->ev)

@@ -8,13 +8,27 @@
 // the wave's quadrant is skipped by the whole wave; so is one whose falloff
 // stays below the splat's alpha threshold at every live pixel of the
 // quadrant inside the box (those pixels are masked by the reference too).
-// A pixel stops at T < T_MIN; the tile stops when no pixel is live.
+// A pixel stops at T < T_MIN; a wave stops when its quadrant has no live
+// pixel, the tile when no pixel is live.
 //
 // Per-pixel arithmetic is raster_tile's (src/render/cpu.rs:609-719) in the
 // same operation order, with pixel offsets built from the 16-aligned tile
 // origin exactly as its 4-lane groups build them
 // (dx = lane + (4*group + ((ox + 0.5) - x))), and exp_neg's polynomial, so
 // the output bytes equal the CPU backend's.
+//
+// Modes:
+//   BM_RGB     one list per tile, RGB8 output.
+//   BM_DIFF    the differentiable forward: pairs carry emission indices (the
+//              record index is powner[e]); the output is the float image
+//              C + T*bg, and each pixel's final transmittance and the
+//              tile-list position after its last contribution are kept for
+//              the adjoint (pix_T / pix_last, 256 per tile in thread order).
+//   BM_FIRST   the first list of the front-to-back split: a tile whose pixels
+//              all saturate writes its RGB8 output and is marked done; any
+//              other tile keeps (T, C) per pixel in pstate for BM_SECOND.
+//   BM_SECOND  the continuation list of the tiles not done: resumes from
+//              pstate and writes the RGB8 output.
 #include "sp_device.h"
 #include "sp_launch.h"
 
@@ -23,23 +37,20 @@ namespace {
 
 constexpr int XCDS = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs
 
-// DIFF: the differentiable forward. Pairs carry emission indices (the record
-// index is powner[e]); the output is the float image C + T*bg, and each
-// pixel's final transmittance and the tile-list position after its last
-// contribution are kept for the adjoint (pix_T / pix_last, 256 per tile in
-// thread order).
-template <int BATCH, bool DIFF>
+template <int BATCH, int MODE>
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
                                                          uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
                                                          const uint32_t *__restrict__ powner,
                                                          const uint4 *__restrict__ rec, uint32_t per_xcd,
                                                          uint32_t *__restrict__ counters, float *__restrict__ pix_T,
-                                                         uint32_t *__restrict__ pix_last) {
+                                                         uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
+                                                         float4 *__restrict__ pstate) {
     __shared__ uint4 s_rec[BATCH][3];
     // Consecutive tiles (rows of an image) on the same XCD share its L2.
     const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
     if (gt >= total_tiles) return;
+    if (MODE == BM_SECOND && done[gt]) return;  // finished by the first list
     const uint32_t lo = view_of_tile(views, nviews, gt);
     const ViewCtx &v = views[lo].ctx;
     uint8_t *__restrict__ rgb = views[lo].out;
@@ -51,20 +62,31 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     const uint32_t qx0 = ox + 8u * (w & 1u), qy0 = oy + 8u * (w >> 1);
     const uint32_t px = qx0 + (lane_i & 7u), py = qy0 + (lane_i >> 3);
     const uint32_t lx = px - ox, ly = py - oy;
-    bool live = px < v.w && py < v.h;
+    const bool inside = px < v.w && py < v.h;
+    bool live = inside;
     const float fx = (float)ox;
     const float gx = (float)(lx & ~3u);  // 4 * group of the reference's 4-lane groups
     const float lane = (float)(lx & 3u);
     const float pyc = ((float)oy + (float)ly) + 0.5f;
+    const uint64_t pix = (uint64_t)gt * BLEND_THREADS + threadIdx.x;
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    if (MODE == BM_SECOND) {
+        const float4 st = pstate[pix];
+        T = st.x;
+        cr = st.y;
+        cg = st.z;
+        cb = st.w;
+        live = inside && !(T < T_MIN);
+    }
     uint32_t last = 0;
+    bool finished = false;
     const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
     uint32_t b = start;
     for (; b < end; b += BATCH) {
         const uint32_t idx = b + threadIdx.x;
         if (threadIdx.x < BATCH && idx < end) {
             uint32_t id = sval[idx];
-            if (DIFF) id = powner[id];
+            if (MODE == BM_DIFF) id = powner[id];
             const uint4 *r = rec + 3 * (uint64_t)id;
             s_rec[threadIdx.x][0] = r[0];
             s_rec[threadIdx.x][1] = r[1];
@@ -102,27 +124,38 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                     cg = cg + __uint_as_float(r1.w) * wgt;
                     cb = cb + __uint_as_float(r2.x) * wgt;
                     T = T * (1.0f - alpha);
-                    if (DIFF) last = (b - start) + k + 1;
+                    if (MODE == BM_DIFF) last = (b - start) + k + 1;
                     if (T < T_MIN) live = false;
                 }
             }
         }
         if (__syncthreads_count(live) == 0) {
             b += BATCH;
+            finished = true;
             break;
         }
     }
     if (threadIdx.x == 0 && end > start) atomicAdd(&counters[CNT_STAGED], min(b, end) - start);
-    if (DIFF) {
-        pix_T[(uint64_t)gt * BLEND_THREADS + threadIdx.x] = T;
-        pix_last[(uint64_t)gt * BLEND_THREADS + threadIdx.x] = last;
-        if (px < v.w && py < v.h) {
+    if (MODE == BM_FIRST) {
+        // `finished` is block-uniform, so the count below is reached by all threads or none.
+        if (!finished) finished = __syncthreads_count(live) == 0;
+        if (!finished) {
+            pstate[pix] = make_float4(T, cr, cg, cb);
+            if (threadIdx.x == 0) done[gt] = 0u;
+            return;
+        }
+        if (threadIdx.x == 0) done[gt] = 1u;
+    }
+    if (MODE == BM_DIFF) {
+        pix_T[pix] = T;
+        pix_last[pix] = last;
+        if (inside) {
             float *o = views[lo].fimg + 3 * ((uint64_t)py * v.w + px);
             o[0] = cr + T * v.bg[0];
             o[1] = cg + T * v.bg[1];
             o[2] = cb + T * v.bg[2];
         }
-    } else if (px < v.w && py < v.h) {
+    } else if (inside) {
         uint8_t *o = rgb + 3 * ((uint64_t)py * v.w + px);
         o[0] = (uint8_t)to_u8(cr + T * v.bg[0]);
         o[1] = (uint8_t)to_u8(cg + T * v.bg[1]);
@@ -130,44 +163,55 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     }
 }
 
-template <bool DIFF>
-hipError_t launch_blend_t(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                          const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
-                          uint32_t *counters, float *pix_T, uint32_t *pix_last, int batch) {
-    if (total_tiles == 0) return hipSuccess;
+template <int MODE>
+void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                 const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                 uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate, int batch) {
     const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
     const dim3 grid(per_xcd * XCDS), block(BLEND_THREADS);
     switch (batch) {
     case 64:
-        hipLaunchKernelGGL((k_blend<64, DIFF>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last);
+        hipLaunchKernelGGL((k_blend<64, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
+                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     case 128:
-        hipLaunchKernelGGL((k_blend<128, DIFF>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last);
+        hipLaunchKernelGGL((k_blend<128, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
+                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     default:
-        hipLaunchKernelGGL((k_blend<256, DIFF>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last);
+        hipLaunchKernelGGL((k_blend<256, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
+                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     }
-    return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_blend(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters,
+hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                        uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch) {
-    return launch_blend_t<false>(s, views, nviews, total_tiles, tile_start, sval, nullptr, rec, counters, nullptr,
-                                 nullptr, batch);
-}
-
-hipError_t launch_blend_diff(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                             const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
-                             const uint4 *rec, uint32_t *counters, float *pix_T, uint32_t *pix_last, int batch) {
-    return launch_blend_t<true>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T,
-                                pix_last, batch);
+    if (total_tiles == 0) return hipSuccess;
+    float4 *ps = reinterpret_cast<float4 *>(pstate);
+    switch (mode) {
+    case BM_DIFF:
+        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T, pix_last,
+                             done, ps, batch);
+        break;
+    case BM_FIRST:
+        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T,
+                              pix_last, done, ps, batch);
+        break;
+    case BM_SECOND:
+        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T,
+                               pix_last, done, ps, batch);
+        break;
+    default:
+        launch_mode<BM_RGB>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T, pix_last,
+                            done, ps, batch);
+        break;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace sp

@@ -27,10 +27,26 @@ constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
 // Device counter slots (uint32 each) in the per-handle counter block.
 enum Counter : int {
-    CNT_PAIRS = 1,     // (splat, tile) pairs over all views, exact even when over capacity
-    CNT_OVERFLOW = 2,  // 1 when the pairs exceeded the pair capacity
-    CNT_STAGED = 3,    // pair records the blend gathered (early termination skips the rest)
+    CNT_PAIRS = 1,     // pairs of the (first) emission phase, exact even when over capacity
+    CNT_OVERFLOW = 2,  // 1 when an emission phase exceeded the pair capacity
+    CNT_STAGED = 3,    // pair records the blends gathered (early termination skips the rest)
+    CNT_PAIRS2 = 4,    // pairs of the second emission phase
     CNT_SLOTS = 16
+};
+
+// Blend modes (blend.hip).
+constexpr int BM_RGB = 0;
+constexpr int BM_DIFF = 1;
+constexpr int BM_FIRST = 2;
+constexpr int BM_SECOND = 3;
+
+// Emission phases (front-to-back split of each view's depth order).
+constexpr int EMIT_FIRST = 0;   // the nearest K1 splats of each view, every tile
+constexpr int EMIT_SECOND = 1;  // the remaining splats, unfinished tiles only
+// K1 = min(vis, max(kmin, ceil(vis / 2^shift))); shift 0 emits everything at once.
+struct EmitSplit {
+    uint32_t kmin;
+    uint32_t shift;
 };
 
 // ProjectCtx (src/render/mod.rs:155-197) plus frame/tile geometry.

@@ -48,32 +48,38 @@ hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2]
 // ---- tiles.hip: (b) emission, (d) ranges ----
 uint32_t emit_blocks(uint32_t n);
 // One (global tile id, record index) pair per overlapped 16x16 tile of each
-// view's depth-ordered splats, views in order. bcnt: nviews rows of
-// emit_blocks(n) block counts; vtot: nviews entries. Sets counters[CNT_PAIRS]
-// (exact) and counters[CNT_OVERFLOW] when above pair_cap. With powner set,
-// pval[e] = e and powner[e] = record index instead.
-hipError_t launch_emit(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                       const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, uint32_t *bcnt,
-                       uint32_t *vtot, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
+// view's depth-ordered splats, views in order, for emission phase `phase`
+// (EMIT_FIRST: the nearest K1 splats of each view, every tile; EMIT_SECOND:
+// the rest, only tiles with ip's row prefix counting them unfinished).
+// bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
+// entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
+// (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
+// also writes each view's unsplit pair count to view_pairs[v]. With powner
+// set, pval[e] = e and powner[e] = record index instead.
+hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
+                       const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, EmitSplit split,
+                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
+                       uint32_t *view_pairs, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
                        uint32_t *counters);
-// tile_start[0..num_tiles] from the tile-sorted pair keys.
-hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *counters, uint64_t pair_cap,
+// Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every view.
+hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
+                             uint32_t *ip);
+// tile_start[0..num_tiles] from the tile-sorted pair keys; *count pairs.
+hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *count, uint64_t pair_cap,
                          uint32_t num_tiles, uint32_t *tile_start);
-// view_pairs[v] = pairs of view v (from the tile ranges).
-hipError_t launch_view_pairs(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                             const uint32_t *tile_start, uint32_t *view_pairs);
 
 // ---- blend.hip: (e) front-to-back blend of every tile of every view ----
+// mode BM_RGB / BM_FIRST / BM_SECOND write RGB8 (ViewParams::out); BM_DIFF
+// takes emission-index pairs (record = powner[e]) and writes the float image
+// (ViewParams::fimg) and the per-pixel final transmittance / last
+// contributing list position (pix_T / pix_last, total_tiles * 256 each).
+// BM_FIRST sets done[t] per tile and keeps unfinished tiles' pixel state in
+// pstate (total_tiles * 256 * 4 floats); BM_SECOND resumes from it.
 // batch = records staged through LDS per step (64, 128 or 256).
-hipError_t launch_blend(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec, uint32_t *counters,
+hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                        uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch);
-// Differentiable forward: sval holds emission indices (record = powner[e]);
-// writes each view's float image (ViewParams::fimg) and the per-pixel final
-// transmittance / last contributing list position (total_tiles * 256 each).
-hipError_t launch_blend_diff(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                             const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
-                             const uint4 *rec, uint32_t *counters, float *pix_T, uint32_t *pix_last, int batch);
 
 // ---- diff.hip: adjoint of the blend and of the projection ----
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission
