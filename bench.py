@@ -42,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 REC_BYTES = 48  # projected record gathered by the blend per pair
 IDX_BYTES = 4  # pair value (scene index) read by the blend
 SPLAT_IN_BYTES = {0: 56, 1: 92, 2: 152, 3: 236}  # f32 bytes read per splat by projection (incl. k)
+F32_PEAK_TFLOPS = 157.3
+# f32 operations of one pixel evaluation of raster_tile: offsets and power (12),
+# exp_neg (24: clamp, range reduction, degree-6 unfused polynomial, scaling),
+# alpha and its tests (4), accumulation and transmittance (8).
+BLEND_FLOPS = 48
 
 
 def log(msg: str) -> None:
@@ -65,6 +70,9 @@ def main() -> None:
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--diff-splats", type=int, default=500_000)
     ap.add_argument("--gradcheck-params", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB device buffer 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -79,6 +87,15 @@ def main() -> None:
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
+    if args.calibrate_copy:
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 5}))
+        return
 
     W, H = args.width, args.height
     log(f"generating {args.splats} SH-3 splats")
@@ -88,6 +105,12 @@ def main() -> None:
     out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
     bg = (0.0, 0.0, 0.0)
 
+    if args.no_autotune:
+        tuned = r.tuning()
+    else:
+        log("autotune on one orbit view")
+        tuned = r.autotune(views[:1], [(W, H)], bg, reps=5)
+    log(f"tuning {tuned}")
     log("warmup")
     for i in range(max(args.warmup, 1)):
         r.render_device(views[i % len(views)], W, H, bg, out.data_ptr())
@@ -145,6 +168,7 @@ def main() -> None:
         "config": {
             "workload": f"plain forward: {args.splats} random SH-3 splats, {W}x{H}, 40-frame orbit at pitch 20 deg",
             "splats": args.splats, "sh_degree": 3, "width": W, "height": H, "frames": 40, "tile": 16,
+            "tuning": {k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")},
         },
         "stages_ms": {"project_sort_emit": round(proj_ms, 4), "tile_sort_ranges": round(sort_ms, 4),
                       "blend": round(blend_ms, 4)},
@@ -177,6 +201,19 @@ def main() -> None:
             result["cpu_baseline"] = cpu_baseline(S, scene, views, W, H, r, args.cpu_seconds)
         except Exception as e:
             result["cpu_baseline"] = {"error": repr(e)}
+    cb = result.get("cpu_baseline", {})
+    if "blend_evaluations_view0" in cb and blend_ms > 0:
+        # Secondary ceiling of the blend: f32 vector issue. E = live in-box pixel
+        # evaluations the reference blend performs for view 0 (counted by the
+        # oracle); each costs BLEND_FLOPS f32 operations in the reference's order.
+        first_ms = stats[0].raster_ms
+        ev = cb["blend_evaluations_view0"]
+        tf = ev * BLEND_FLOPS / (first_ms * 1e-3) / 1e12 if first_ms > 0 else 0.0
+        result["valu_roofline"] = {
+            "kernel": "k_blend", "bound": "f32 vector", "evaluations": ev, "flops_per_evaluation": BLEND_FLOPS,
+            "achieved": round(tf, 2), "peak": F32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / F32_PEAK_TFLOPS, 4), "view": 0,
+        }
     r.close()
     del r, scene
     if not args.no_diff:
@@ -351,7 +388,8 @@ def bench_batched(S, torch, scene, nviews: int) -> dict:
     outs = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for (w, h) in sizes]
     ptrs = [o.data_ptr() for o in outs]
     bg = (0.0, 0.0, 0.0)
-    log(f"batched: {nviews} views")
+    log(f"batched: {nviews} views, autotune")
+    tuned = r.autotune(views, sizes, bg, reps=3)
     r.render_batch(views, sizes, bg, ptrs, on_device=True)  # warm (captures)
     torch.cuda.synchronize()
     reps = 3
@@ -376,6 +414,7 @@ def bench_batched(S, torch, scene, nviews: int) -> dict:
         "plain_sequential_views_per_s": round(nviews / ts, 3),
         "speedup_vs_plain": round(ts / tb, 3),
         "pixels": int(sum(w * h for w, h in sizes)),
+        "tuning": {k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")},
     }
 
 
@@ -388,7 +427,7 @@ def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:
     cores = host_cores()
     cpu = O.CpuTiled(scene, threads=cores)
     log(f"cpu baseline on {cores} threads")
-    img, _ = cpu.render(views[0], W, H, (0, 0, 0))  # warm
+    img, _, evals = cpu.render(views[0], W, H, (0, 0, 0), work=True)  # warm; blend work of view 0
     hip_img, _ = r.render(views[0], W, H, (0, 0, 0))
     exact = bool(np.array_equal(img, hip_img.pixels))
     n = 0
@@ -404,6 +443,7 @@ def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:
         "kind": "port",
         "sample": f"first {n} views of the 40-frame orbit, same scene and size ({el:.1f} s)",
         "hip_image_bit_exact_vs_cpu": exact,
+        "blend_evaluations_view0": evals,
     }
 
 

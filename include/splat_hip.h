@@ -178,6 +178,28 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
                     sp_stats *stats);
 
 /* ---------------------------------------------------------------------------
+ * Tuning (no reference counterpart). The tunables change only the speed;
+ * every setting renders the same bytes.
+ * ------------------------------------------------------------------------- */
+typedef struct sp_tuning {
+    int32_t blend_batch;   /* records staged through LDS per blend step: 64, 128 or 256 */
+    uint32_t split_shift;  /* front-to-back split: the first phase takes each view's nearest
+                              max(split_kmin, visible >> split_shift) splats; 0 = no split */
+    uint32_t split_kmin;
+    uint32_t reserved;
+} sp_tuning;
+
+int sp_get_tuning(const sp_handle *h, sp_tuning *t);
+int sp_set_tuning(sp_handle *h, const sp_tuning *t);
+
+/* Autotuner: times candidate tunings on this batch of views (captured batch
+ * replays into handle-owned scratch, median of `reps` after a warm-up,
+ * coordinate descent over split_shift, blend_batch, split_kmin), keeps the
+ * fastest on the handle and reports it with its batch time in milliseconds. */
+int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t n_views,
+                const float background[3], int32_t reps, sp_tuning *best, double *best_ms);
+
+/* ---------------------------------------------------------------------------
  * Differentiable rendering (no reference counterpart: splat-tui renders only;
  * this is the adjoint of the same forward). A handle made by sp_diff_create
  * holds no scene: each sp_diff_forward reads raw, unactivated parameters from

@@ -143,6 +143,15 @@ class SpParamGrads(C.Structure):
     ]
 
 
+class SpTuning(C.Structure):
+    _fields_ = [
+        ("blend_batch", C.c_int32),
+        ("split_shift", C.c_uint32),
+        ("split_kmin", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 # Every symbol include/splat_hip.h declares, with its signature.
 HIP_SYMBOLS = {
     "sp_default_options": (None, [C.POINTER(SpOptions)]),
@@ -164,6 +173,13 @@ HIP_SYMBOLS = {
     "sp_render_batch": (
         C.c_int,
         [C.c_void_p, C.POINTER(SpView), u32p, C.c_uint32, f32p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SpStats)],
+    ),
+    "sp_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(SpTuning)]),
+    "sp_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(SpTuning)]),
+    "sp_autotune": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpView), u32p, C.c_uint32, f32p, C.c_int32, C.POINTER(SpTuning),
+         C.POINTER(C.c_double)],
     ),
     "sp_diff_create": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(SpOptions), C.POINTER(C.c_void_p)]),
     "sp_diff_forward": (

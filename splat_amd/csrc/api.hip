@@ -926,6 +926,117 @@ int sp_render_batch(sp_handle *h, const sp_view *views, const uint32_t *wh, uint
     return SP_OK;
 }
 
+int sp_get_tuning(const sp_handle *h, sp_tuning *t) {
+    if (!h || !t) return fail(SP_E_INVALID, "sp_get_tuning: NULL argument");
+    std::memset(t, 0, sizeof *t);
+    t->blend_batch = h->blend_batch;
+    t->split_shift = h->split_shift;
+    t->split_kmin = h->split_kmin;
+    return SP_OK;
+}
+
+int sp_set_tuning(sp_handle *h, const sp_tuning *t) {
+    if (!h || !t) return fail(SP_E_INVALID, "sp_set_tuning: NULL argument");
+    if (t->blend_batch != 64 && t->blend_batch != 128 && t->blend_batch != 256)
+        return fail(SP_E_INVALID, "blend_batch must be 64, 128 or 256");
+    if (t->split_shift > 16) return fail(SP_E_INVALID, "split_shift must be 0..16");
+    h->blend_batch = t->blend_batch;
+    h->split_shift = t->split_shift;
+    h->split_kmin = t->split_kmin;
+    return SP_OK;
+}
+
+// Coordinate descent over the tunables on the caller's batch: the split depth
+// first (the largest effect), then the LDS batch of the blend, then the
+// minimum size of the first phase. Each candidate is timed as the median of
+// `reps` captured batch replays after one warm-up (which also captures the
+// graph). Every candidate renders the same bytes; only the time differs.
+int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t n_views,
+                const float background[3], int32_t reps, sp_tuning *best, double *best_ms) {
+    if (!h || !views || !wh || !background || n_views == 0) return fail(SP_E_INVALID, "sp_autotune: bad argument");
+    if (h->diff) return fail(SP_E_INVALID, "sp_autotune: the differentiable path has no tunables");
+    int rc = setup(h);
+    if (rc) return rc;
+    std::vector<sp_view> bv;
+    std::vector<uint32_t> bwh;
+    uint64_t bytes = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+        rc = check_frame(wh[2 * v], wh[2 * v + 1]);
+        if (rc) return rc;
+        if (wh[2 * v] == 0 || wh[2 * v + 1] == 0) continue;
+        bv.push_back(views[v]);
+        bwh.push_back(wh[2 * v]);
+        bwh.push_back(wh[2 * v + 1]);
+        bytes += 3ull * wh[2 * v] * wh[2 * v + 1];
+    }
+    const uint32_t nb = (uint32_t)bv.size();
+    if (nb == 0) return fail(SP_E_INVALID, "sp_autotune: every view is empty");
+    rc = ensure_rgb(h, bytes);
+    if (rc) return rc;
+    std::vector<void *> outs(nb);
+    uint64_t o = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+        outs[k] = h->d_rgb + o;
+        o += 3ull * bwh[2 * k] * bwh[2 * k + 1];
+    }
+    reps = std::max(1, std::min(reps, 50));
+    sp_tuning cur;
+    sp_get_tuning(h, &cur);
+    auto time_of = [&](const sp_tuning &t, double *ms) -> int {
+        int r = sp_set_tuning(h, &t);
+        if (r) return r;
+        r = run_views(h, nb, bv.data(), bwh.data(), background, outs.data(), BL_RGB, true, nullptr);
+        if (r) return r;
+        std::vector<double> ts;
+        for (int k = 0; k < reps; ++k) {
+            const double t0 = now_ms();
+            r = run_views(h, nb, bv.data(), bwh.data(), background, outs.data(), BL_RGB, true, nullptr);
+            if (r) return r;
+            ts.push_back(now_ms() - t0);
+        }
+        std::sort(ts.begin(), ts.end());
+        *ms = ts[ts.size() / 2];
+        return SP_OK;
+    };
+    double cur_ms = 0;
+    rc = time_of(cur, &cur_ms);
+    if (rc) return rc;
+    auto consider = [&](sp_tuning t) -> int {
+        double ms = 0;
+        const int r = time_of(t, &ms);
+        if (r) return r;
+        if (ms < cur_ms) {
+            cur = t;
+            cur_ms = ms;
+        }
+        return SP_OK;
+    };
+    for (uint32_t sh : {0u, 2u, 3u, 4u, 5u}) {
+        if (sh == cur.split_shift) continue;
+        sp_tuning t = cur;
+        t.split_shift = sh;
+        if ((rc = consider(t))) return rc;
+    }
+    for (int bb : {64, 128, 256}) {
+        if (bb == cur.blend_batch) continue;
+        sp_tuning t = cur;
+        t.blend_batch = bb;
+        if ((rc = consider(t))) return rc;
+    }
+    if (cur.split_shift > 0)
+        for (uint32_t km : {4096u, 16384u, 65536u}) {
+            if (km == cur.split_kmin) continue;
+            sp_tuning t = cur;
+            t.split_kmin = km;
+            if ((rc = consider(t))) return rc;
+        }
+    rc = sp_set_tuning(h, &cur);
+    if (rc) return rc;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return SP_OK;
+}
+
 int sp_diff_create(uint64_t n, uint32_t sh_degree, const sp_options *opt_in, sp_handle **out) {
     if (!out) return fail(SP_E_INVALID, "sp_diff_create: out is NULL");
     *out = nullptr;

@@ -5,8 +5,8 @@
 // every segment (per-wave LDS histograms merged with global atomics; counts
 // are order-independent, so the result is deterministic). Then, per 8-bit
 // digit:
-//   * upsweep: each block counts the digits of its 4096-key tile into a
-//     (segment, digit, block) table,
+//   * upsweep: each block counts the digits of a 4096-key tile into a
+//     (segment, digit, tile) table,
 //   * digit-row scan: one workgroup per (segment, digit) row turns the counts
 //     into global positions (row base = the histogram's exclusive prefix),
 //   * downsweep: each block re-reads its tile, ranks it with a wave64 ballot
@@ -15,6 +15,8 @@
 //     follow key order: stable), exchanges it through LDS into digit order and
 //     writes runs of one digit to consecutive addresses.
 // No block ever waits for another (no lookback chains across the 8 XCDs).
+// Grids are bounded and blocks stride over the live tiles (the live key count
+// is read on the device), so sorting a few keys in a large buffer is cheap.
 //
 // The rasterizer sorts twice per view: the visible splats by depth bits (32
 // bits, 4 passes; stable, so equal depths keep scene-index order), then the
@@ -35,6 +37,7 @@ namespace {
 constexpr int NWAVES = RADIX_THREADS / 64;
 constexpr int WAVE_KEYS = RADIX_ITEMS * 64;  // 1024 keys per wave per tile
 constexpr int HIST_ITEMS = 16;
+constexpr uint32_t RADIX_GRID = 4096;  // bound on the upsweep / downsweep grid
 
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t *__restrict__ keys,
                                                               const uint32_t *__restrict__ seg_counts,
@@ -61,47 +64,49 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t *__
     }
 }
 
-// table[(seg * 256 + d) * bps + local] = count of digit d in block `local` of `seg`.
+// Live 4096-key tiles of a segment.
+__device__ __forceinline__ uint32_t live_tiles(uint64_t n) { return (uint32_t)((n + RADIX_TILE - 1) / RADIX_TILE); }
+
+// table[(seg * 256 + d) * bps + t] = count of digit d in tile t of `seg`.
+// gps workgroups per segment stride over the segment's live tiles, so the
+// grid follows the capacity only up to a bound and idle work follows the
+// live count.
 __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ seg_counts,
                                                            uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
-                                                           int shift, uint32_t *__restrict__ table) {
+                                                           uint32_t gps, int shift, uint32_t *__restrict__ table) {
     __shared__ uint32_t s_h[NWAVES][RADIX_BINS];
-    const uint32_t seg = blockIdx.x / bps, local = blockIdx.x % bps;
+    const uint32_t seg = blockIdx.x / gps;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    for (uint32_t k = threadIdx.x; k < NWAVES * RADIX_BINS; k += RADIX_THREADS) (&s_h[0][0])[k] = 0;
-    __syncthreads();
     const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
-    const uint64_t wbase = (uint64_t)local * RADIX_TILE + (uint64_t)w * WAVE_KEYS;
+    const uint32_t tiles = live_tiles(n);
     const uint32_t *ks = kin + seg * seg_stride;
+    for (uint32_t local = blockIdx.x % gps; local < tiles; local += gps) {
+        for (uint32_t k = threadIdx.x; k < NWAVES * RADIX_BINS; k += RADIX_THREADS) (&s_h[0][0])[k] = 0;
+        __syncthreads();
+        const uint64_t wbase = (uint64_t)local * RADIX_TILE + (uint64_t)w * WAVE_KEYS;
 #pragma unroll 4
-    for (int k = 0; k < RADIX_ITEMS; ++k) {
-        const uint64_t idx = wbase + (uint64_t)k * 64 + lane;
-        if (idx < n) atomicAdd(&s_h[w][(ks[idx] >> shift) & 0xFFu], 1u);
-    }
-    __syncthreads();
-    const uint32_t d = threadIdx.x;
-    uint32_t c = 0;
+        for (int k = 0; k < RADIX_ITEMS; ++k) {
+            const uint64_t idx = wbase + (uint64_t)k * 64 + lane;
+            if (idx < n) atomicAdd(&s_h[w][(ks[idx] >> shift) & 0xFFu], 1u);
+        }
+        __syncthreads();
+        const uint32_t d = threadIdx.x;
+        uint32_t c = 0;
 #pragma unroll
-    for (int ww = 0; ww < NWAVES; ++ww) c += s_h[ww][d];
-    table[((uint64_t)seg * RADIX_BINS + d) * bps + local] = c;
+        for (int ww = 0; ww < NWAVES; ++ww) c += s_h[ww][d];
+        table[((uint64_t)seg * RADIX_BINS + d) * bps + local] = c;
+        __syncthreads();
+    }
 }
 
-__global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__restrict__ kin,
-                                                             const uint32_t *__restrict__ vin,
-                                                             uint32_t *__restrict__ kout,
-                                                             uint32_t *__restrict__ vout,
-                                                             const uint32_t *__restrict__ seg_counts,
-                                                             uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
-                                                             int shift, const uint32_t *__restrict__ table) {
-    __shared__ uint32_t s_wh[NWAVES][RADIX_BINS];  // per-wave digit counters, then per-wave offsets
-    __shared__ uint32_t s_lstart[RADIX_BINS];      // block-local start of each digit
-    __shared__ uint32_t s_gbase[RADIX_BINS];       // segment position minus local position
-    __shared__ uint2 s_ex[RADIX_TILE];             // key/value exchange in digit order
-    const uint32_t seg = blockIdx.x / bps, local = blockIdx.x % bps;
-    const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
+__device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                               uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                               uint32_t seg, uint64_t seg_stride, uint64_t n, uint32_t bps,
+                                               uint32_t local, int shift, const uint32_t *__restrict__ table,
+                                               uint32_t (*s_wh)[RADIX_BINS], uint32_t *s_lstart, uint32_t *s_gbase,
+                                               uint2 *s_ex) {
     const uint64_t base = (uint64_t)local * RADIX_TILE;
-    if (base >= n) return;  // uniform per block
     const uint32_t *ks = kin + seg * seg_stride;
     const uint32_t *vs = vin + seg * seg_stride;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -172,6 +177,29 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
     }
 }
 
+__global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__restrict__ kin,
+                                                             const uint32_t *__restrict__ vin,
+                                                             uint32_t *__restrict__ kout,
+                                                             uint32_t *__restrict__ vout,
+                                                             const uint32_t *__restrict__ seg_counts,
+                                                             uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
+                                                             uint32_t gps, int shift,
+                                                             const uint32_t *__restrict__ table) {
+    __shared__ uint32_t s_wh[NWAVES][RADIX_BINS];  // per-wave digit counters, then per-wave offsets
+    __shared__ uint32_t s_lstart[RADIX_BINS];      // block-local start of each digit
+    __shared__ uint32_t s_gbase[RADIX_BINS];       // segment position minus local position
+    __shared__ uint2 s_ex[RADIX_TILE];             // key/value exchange in digit order
+    const uint32_t seg = blockIdx.x / gps;
+    const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
+    const uint32_t tiles = live_tiles(n);
+    for (uint32_t local = blockIdx.x % gps; local < tiles; local += gps) {
+        downsweep_tile(kin, vin, kout, vout, seg, seg_stride, n, bps, local, shift, table, s_wh, s_lstart, s_gbase,
+                       s_ex);
+        __syncthreads();  // LDS is reused by the next tile
+    }
+}
+
+
 }  // namespace
 
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap) {
@@ -196,14 +224,19 @@ hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2]
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t bps = (uint32_t)((n_cap + RADIX_TILE - 1) / RADIX_TILE);
+    // Workgroups per segment: one per tile up to a grid bound; they stride over
+    // the live tiles, so a capacity far above the live count costs little.
+    const uint32_t gps = std::max<uint32_t>(1, std::min<uint32_t>(bps, std::max<uint32_t>(1, RADIX_GRID / nseg)));
     for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_upsweep, dim3(bps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], seg_counts,
-                           seg_stride, n_cap, bps, 8 * p, table);
+        hipLaunchKernelGGL(k_upsweep, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], seg_counts,
+                           seg_stride, n_cap, bps, gps, 8 * p, table);
         e = hipGetLastError();
-        if (e == hipSuccess) e = launch_scan_digit_rows(s, table, bps, nseg, hist + RADIX_BINS * p, hist_seg_stride);
+        if (e == hipSuccess)
+            e = launch_scan_digit_rows(s, table, bps, nseg, hist + RADIX_BINS * p, hist_seg_stride, seg_counts,
+                                       n_cap);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_downsweep, dim3(bps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], vals[cur],
-                           keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, 8 * p, table);
+        hipLaunchKernelGGL(k_downsweep, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], vals[cur],
+                           keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps, 8 * p, table);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         cur = 1 - cur;

@@ -53,21 +53,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict
     if (row_total && threadIdx.x == 0) row_total[blockIdx.x] = carry - base;
 }
 
-// Radix digit rows: row r = (segment g, digit d) of `row_len` per-block counts;
+// Radix digit rows: row r = (segment g, digit d) of `row_len` per-tile counts,
+// of which the first ceil(min(seg_counts[g], n_cap) / RADIX_TILE) are live;
 // the row's base is the exclusive prefix of the segment's digit histogram
 // hist[g * hist_seg_stride + 0..255] below d.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
                                                                   const uint32_t *__restrict__ hist,
-                                                                  uint32_t hist_seg_stride) {
+                                                                  uint32_t hist_seg_stride,
+                                                                  const uint32_t *__restrict__ seg_counts,
+                                                                  uint64_t n_cap) {
     __shared__ uint32_t s_base;
     const uint32_t g = blockIdx.x / RADIX_BINS, d = blockIdx.x % RADIX_BINS;
     uint32_t tot;
     const uint32_t pre = block_exclusive_scan<SCAN_THREADS>(hist[(uint64_t)g * hist_seg_stride + threadIdx.x], &tot);
     if (threadIdx.x == d) s_base = pre;
     __syncthreads();
+    const uint64_t n = min((uint64_t)seg_counts[g], n_cap);
+    const uint64_t live = min(row_len, (n + RADIX_TILE - 1) / RADIX_TILE);
     uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
     uint32_t carry = s_base;
-    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) carry += scan_chunk(row, off, row_len, carry, true);
+    for (uint64_t off = 0; off < live; off += SCAN_CHUNK) carry += scan_chunk(row, off, live, carry, true);
 }
 
 __global__ __launch_bounds__(256) void k_fill(uint32_t *__restrict__ p, uint64_t n, uint32_t value) {
@@ -92,10 +97,11 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 }
 
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *hist, uint32_t hist_seg_stride) {
+                                  const uint32_t *hist, uint32_t hist_seg_stride, const uint32_t *seg_counts,
+                                  uint64_t n_cap) {
     if (nseg == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len, hist,
-                       hist_seg_stride);
+                       hist_seg_stride, seg_counts, n_cap);
     return hipGetLastError();
 }
 

@@ -15,9 +15,11 @@ namespace sp {
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total);
 // Radix tables: rows (segment g, digit d), based at the exclusive prefix of
-// hist[g * hist_seg_stride + 0..255] below d.
+// hist[g * hist_seg_stride + 0..255] below d; only the live prefix of each row
+// (ceil(min(seg_counts[g], n_cap) / RADIX_TILE) entries) is scanned.
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *hist, uint32_t hist_seg_stride);
+                                  const uint32_t *hist, uint32_t hist_seg_stride, const uint32_t *seg_counts,
+                                  uint64_t n_cap);
 // p[0..n) = value (a kernel, so it may be captured in a graph).
 hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 

@@ -410,7 +410,7 @@ hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nv
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *count, uint64_t pair_cap,
                          uint32_t num_tiles, uint32_t *tile_start) {
     const uint64_t work = pair_cap > num_tiles ? pair_cap : (uint64_t)num_tiles + 1;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((work + 255) / 256, 65536);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((work + 255) / 256, 4096);  // grid-stride over the pairs
     hipLaunchKernelGGL(k_ranges, dim3(grid), dim3(256), 0, s, skey, count, pair_cap, num_tiles, tile_start);
     return hipGetLastError();
 }

@@ -147,6 +147,29 @@ class HipRenderer:
         stats = [RenderStats.from_c(st[i]) for i in range(nv)]
         return (arrays, stats) if not on_device else stats
 
+    def tuning(self) -> dict:
+        """Current tunables (sp_get_tuning): blend_batch, split_shift, split_kmin."""
+        t = N.SpTuning()
+        N.check(self._lib.sp_get_tuning(self._h, C.byref(t)))
+        return {"blend_batch": t.blend_batch, "split_shift": t.split_shift, "split_kmin": t.split_kmin}
+
+    def set_tuning(self, blend_batch: int = 256, split_shift: int = 3, split_kmin: int = 16384) -> None:
+        t = N.SpTuning(blend_batch, split_shift, split_kmin, 0)
+        N.check(self._lib.sp_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, views, sizes, background=(0.0, 0.0, 0.0), reps: int = 3) -> dict:
+        """Pick the fastest tunables for this batch of views (sp_autotune); returns them
+        with the batch time in milliseconds."""
+        nv = len(views)
+        va = (N.SpView * nv)(*views)
+        wh = np.ascontiguousarray(np.array(sizes, np.uint32).reshape(nv, 2))
+        t = N.SpTuning()
+        ms = C.c_double(0.0)
+        N.check(self._lib.sp_autotune(self._h, va, wh.ctypes.data_as(N.u32p), nv, N.fptr(_bg(background)), reps,
+                                      C.byref(t), C.byref(ms)))
+        return {"blend_batch": t.blend_batch, "split_shift": t.split_shift, "split_kmin": t.split_kmin,
+                "batch_ms": ms.value}
+
     def frame_info(self) -> dict:
         """Work counters of the last rendered view (sp_last_frame_info)."""
         fi = N.SpFrameInfo()

@@ -82,3 +82,45 @@ def test_batch_on_device_outputs():
     for k, (v, (w, h)) in enumerate(zip(views, sizes)):
         one, _ = r.render(v, w, h, (0, 0, 0))
         assert np.array_equal(outs[k].cpu().numpy().reshape(h, w, 3), one.pixels)
+
+
+@pytest.mark.parametrize("tuning", [
+    {"blend_batch": 256, "split_shift": 0, "split_kmin": 16384},
+    {"blend_batch": 64, "split_shift": 2, "split_kmin": 0},
+    {"blend_batch": 128, "split_shift": 5, "split_kmin": 4096},
+    {"blend_batch": 256, "split_shift": 16, "split_kmin": 1},
+    {"blend_batch": 64, "split_shift": 0, "split_kmin": 0},
+])
+def test_every_tuning_renders_the_same_bytes(oracle, tuning):
+    """The tunables (LDS batch, front-to-back split depth) change only the speed: single
+    and batched renders stay byte-identical to the cpu backend, including splits so deep
+    that most tiles finish in the second phase."""
+    s = S.random_scene(40000, seed=24, sh_degree=3)
+    views, sizes = S.random_views(s, 6, seed=8, sizes=((320, 180), (640, 360), (97, 53)))
+    r = S.HipRenderer(s)
+    r.set_tuning(**tuning)
+    cpu = oracle.CpuTiled(s)
+    case(f"tuning {tuning}")
+    imgs, stats = r.render_batch(views, sizes, (0.1, 0.2, 0.3))
+    for k, (v, (w, h)) in enumerate(zip(views, sizes)):
+        ref, rst = cpu.render(v, w, h, (0.1, 0.2, 0.3))
+        assert np.array_equal(imgs[k], ref), f"view {k}"
+        assert (stats[k].visible, stats[k].overlaps) == (rst.visible, rst.overlaps)
+        one, st = r.render(v, w, h, (0.1, 0.2, 0.3))
+        assert np.array_equal(one.pixels, ref), f"single view {k}"
+        assert st.overlaps == rst.overlaps
+
+
+def test_autotune_picks_a_tuning_and_keeps_bytes(oracle):
+    s = S.random_scene(50000, seed=25, sh_degree=3)
+    views, sizes = S.random_views(s, 8, seed=9, sizes=((320, 180), (640, 360)))
+    r = S.HipRenderer(s)
+    case("autotune")
+    best = r.autotune(views, sizes, (0, 0, 0), reps=2)
+    print(f"[tune] {best}", flush=True)
+    assert best["blend_batch"] in (64, 128, 256) and best["batch_ms"] > 0
+    assert r.tuning()["split_shift"] == best["split_shift"]
+    imgs, _ = r.render_batch(views, sizes, (0, 0, 0))
+    cpu = oracle.CpuTiled(s)
+    for k, (v, (w, h)) in enumerate(zip(views, sizes)):
+        assert np.array_equal(imgs[k], cpu.render(v, w, h, (0, 0, 0))[0]), f"view {k}"
