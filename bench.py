@@ -376,6 +376,13 @@ def bench_precision(S, torch, args, W, H) -> dict:
             "image_psnr_vs_fp64": round(float(S.psnr(img.pixels, img64)), 3),
             "image_max_abs_err_vs_fp64": int(d.max()),
         }
+        if storage != "f32":
+            # The renderer's own error: against an FP64 render of the parameters the
+            # 16-bit storage actually holds.
+            _, img_r = O.render_f64(scene.storage_rounded(storage), v0, W, H, (0, 0, 0), threads=cores)
+            dr = np.abs(img.pixels.astype(int) - img_r.astype(int))
+            out["storages"][storage]["image_psnr_vs_fp64_of_rounded_scene"] = round(float(S.psnr(img.pixels, img_r)), 3)
+            out["storages"][storage]["image_max_abs_err_vs_fp64_of_rounded_scene"] = int(dr.max())
         r.close()
     return out
 

@@ -116,6 +116,33 @@ class Scene:
         )
         return s
 
+    def storage_rounded(self, storage: str) -> "Scene":
+        """The scene as the FP16 / BF16 device storage holds it: scales, colours and SH
+        rounded to 16 bits (round to nearest even, sp_to_f16 / sp_to_bf16), quaternions
+        rounded and renormalised; positions and opacities stay f32. Covariances are
+        recomputed from the rounded factors."""
+        if storage not in ("f16", "bf16"):
+            raise ValueError("storage must be 'f16' or 'bf16'")
+        L = N.scene_lib()
+        conv = L.sp_to_f16 if storage == "f16" else L.sp_to_bf16
+
+        def rnd(a):
+            a = np.ascontiguousarray(a, np.float32)
+            h = np.zeros(a.size, np.uint16)
+            conv(N.fptr(a.reshape(-1)), a.size, h.ctypes.data_as(N.u16p))
+            if storage == "f16":
+                return h.view(np.float16).astype(np.float32).reshape(a.shape)
+            return (h.astype(np.uint32) << 16).view(np.float32).reshape(a.shape)
+
+        q = rnd(self.rotations).astype(np.float64)
+        q /= np.maximum(np.linalg.norm(q, axis=1, keepdims=True), 1e-30)
+        s = Scene(
+            positions=self.positions.copy(), scales=rnd(self.scales), rotations=q.astype(np.float32),
+            opacities=self.opacities.copy(), colors=rnd(self.colors), sh_degree=self.sh_degree,
+            sh_rest=None if self.sh_rest is None else rnd(self.sh_rest), meta=dict(self.meta),
+        )
+        return s.finalize()
+
 
 def empty_scene() -> Scene:
     z3 = np.zeros((0, 3), np.float32)
