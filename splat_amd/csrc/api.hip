@@ -123,6 +123,7 @@ struct sp_handle {
     // Per-view arrays, view_cap views of n entries each.
     uint32_t view_cap = 0;
     uint4 *d_rec = nullptr;
+    uint2 *d_box = nullptr;  // depth-ordered pixel boxes, view_cap x n
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
     uint32_t *d_vstat = nullptr;  // 3 x view_cap: visible, pairs, emission view bases
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
@@ -258,6 +259,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     const uint64_t old = h->view_cap;
     const uint64_t blocks = std::max(sp::project_blocks(h->n), sp::emit_blocks(h->n));
     dfree(h, h->d_rec, old * nn * 48);
+    dfree(h, h->d_box, old * nn * 8);
     for (int b = 0; b < 2; ++b) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
@@ -274,6 +276,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     h->view_cap = 0;
     const uint64_t v = views;
     int rc = dalloc(h, (void **)&h->d_rec, v * nn * 48);
+    if (!rc) rc = dalloc(h, (void **)&h->d_box, v * nn * 8);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
@@ -417,9 +420,9 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     int di = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_hist, h->d_table, &di));
     h->depth_buf = di;
-    HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, es, nullptr,
-                            h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0], powner,
-                            h->pair_cap, h->d_counters));
+    HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box, h->d_views,
+                            es, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0],
+                            powner, h->pair_cap, h->d_counters));
     HIP_TRY(mark(1));
     int pi = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
@@ -436,9 +439,9 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     HIP_TRY(mark(3));
     if (split) {
         HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip));
-        HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_views, es,
-                                h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0],
-                                nullptr, h->pair_cap, h->d_counters));
+        HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box,
+                                h->d_views, es, h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0],
+                                h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));
     if (split) {
@@ -689,6 +692,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_soa);
     (void)hipFree(h->d_soa16);
     (void)hipFree(h->d_rec);
+    (void)hipFree(h->d_box);
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(h->d_dkey[b]);
         (void)hipFree(h->d_dval[b]);

@@ -58,9 +58,10 @@ uint32_t emit_blocks(uint32_t n);
 // (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
 // also writes each view's unsplit pair count to view_pairs[v]. With powner
 // set, pval[e] = e and powner[e] = record index instead.
+// box: n entries per view, written by FIRST (the depth-ordered pixel boxes) and read by SECOND.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                       const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, EmitSplit split,
-                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
+                       const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                       EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
                        uint32_t *view_pairs, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
                        uint32_t *counters);
 // Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every view.

@@ -49,15 +49,16 @@ struct Rect {
     uint32_t tx0, ty0, tx1, ty1;
 };
 
-// Tile rectangle of the j-th depth-ordered splat of view v.
-__device__ __forceinline__ Rect splat_rect(const uint32_t *__restrict__ dval, const uint4 *__restrict__ rec,
+// Tile rectangle of the j-th depth-ordered splat of view v, from the
+// depth-ordered pixel boxes (box[vbase + j] = the record's packed x0|y0, x1|y1).
+__device__ __forceinline__ Rect splat_rect(const uint32_t *__restrict__ dval, const uint2 *__restrict__ box,
                                            uint64_t vbase, uint32_t j, uint32_t nvis, uint32_t tiles_x,
                                            uint32_t tile_base) {
     Rect r{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
     if (j < nvis) {
         r.id = (uint32_t)(vbase + dval[vbase + j]);
-        const uint4 r2 = rec[3 * (uint64_t)r.id + 2];
-        const uint32_t x0 = r2.y & 0xFFFFu, y0 = r2.y >> 16, x1 = r2.z & 0xFFFFu, y1 = r2.z >> 16;
+        const uint2 b = box[vbase + j];
+        const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
         if (x0 < x1 && y0 < y1) {
             r.tx0 = x0 / TILE;
             r.ty0 = y0 / TILE;
@@ -105,7 +106,7 @@ __device__ __forceinline__ uint32_t phase_count(const Rect &r, uint32_t j, uint3
 template <int PHASE>
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                              uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                             const uint4 *__restrict__ rec,
+                                                             const uint2 *__restrict__ box,
                                                              const ViewParams *__restrict__ views, EmitSplit split,
                                                              const uint32_t *__restrict__ ip,
                                                              uint32_t *__restrict__ bcnt,
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
     const uint32_t k1 = first_count(nvis, split);
     const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
     const uint32_t j = local * EMIT_THREADS + threadIdx.x;
-    const Rect r = splat_rect(dval, rec, vbase, j, nvis, tiles_x, tile_base);
+    const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
     uint32_t total;
     block_exclusive_scan<EMIT_THREADS>(phase_count<PHASE>(r, j, k1, ip, tile_base, tiles_x), &total);
     if (threadIdx.x == 0) bcnt[blockIdx.x] = total;
@@ -125,6 +126,24 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
         block_exclusive_scan<EMIT_THREADS>(r.cnt, &all);
         if (threadIdx.x == 0) bcnt_all[blockIdx.x] = all;
     }
+}
+
+// box[v*n + j] = pixel box words of view v's j-th depth-ordered splat: one
+// random 8-byte gather per visible splat, so that the emission passes read
+// the boxes in depth order instead of gathering them from the 48-byte records.
+__global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *__restrict__ dval, uint32_t n_splats,
+                                                               uint32_t bpv, const uint32_t *__restrict__ vis_count,
+                                                               const uint4 *__restrict__ rec,
+                                                               uint2 *__restrict__ box) {
+    const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
+    const uint64_t vbase = (uint64_t)v * n_splats;
+    const uint32_t nvis = min(vis_count[v], n_splats);
+    const uint32_t j = local * EMIT_THREADS + threadIdx.x;
+    if (j >= nvis) return;
+    const uint2 *r = reinterpret_cast<const uint2 *>(rec + 3 * (vbase + dval[vbase + j]) + 2);
+    const uint2 w01 = r[0];  // words 8-9 of the record: blue, x0|y0
+    const uint2 w23 = r[1];  // words 10-11: x1|y1, threshold
+    box[vbase + j] = make_uint2(w01.y, w23.x);
 }
 
 // Pair-balanced expansion: the phase's pairs (views concatenated) are cut
@@ -138,7 +157,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                               uint32_t bpv, uint32_t nviews,
                                                               const uint32_t *__restrict__ vis_count,
-                                                              const uint4 *__restrict__ rec,
+                                                              const uint2 *__restrict__ box,
                                                               const ViewParams *__restrict__ views, EmitSplit split,
                                                               const uint32_t *__restrict__ boff,
                                                               const uint32_t *__restrict__ vbase_pairs,
@@ -191,7 +210,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
             const uint32_t k1 = first_count(nvis, split);
             const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
             const uint32_t j = blk * EMIT_THREADS + threadIdx.x;
-            const Rect r = splat_rect(dval, rec, vbase, j, nvis, tiles_x, tile_base);
+            const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
             uint32_t total;
             const uint32_t lscan =
                 block_exclusive_scan<EMIT_THREADS>(phase_count<EMIT_FIRST>(r, j, k1, nullptr, tile_base, tiles_x), &total);
@@ -242,7 +261,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
 // splat by binary search over the in-block offsets.
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                                    uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                                   const uint4 *__restrict__ rec,
+                                                                   const uint2 *__restrict__ box,
                                                                    const ViewParams *__restrict__ views,
                                                                    EmitSplit split, const uint32_t *__restrict__ ip,
                                                                    const uint32_t *__restrict__ boff,
@@ -261,7 +280,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
     const uint32_t k1 = first_count(nvis, split);
     const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
     const uint32_t j = local * EMIT_THREADS + threadIdx.x;
-    const Rect r = splat_rect(dval, rec, vbase, j, nvis, tiles_x, tile_base);
+    const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
     uint32_t total;
     const uint32_t lscan =
         block_exclusive_scan<EMIT_THREADS>(phase_count<EMIT_SECOND>(r, j, k1, ip, tile_base, tiles_x), &total);
@@ -358,13 +377,20 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
 
 template <int PHASE>
 hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                      const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, EmitSplit split,
+                      const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                      EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t count_slot,
                       uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
+    if (PHASE == EMIT_FIRST) {
+        hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+                           vis_count, rec, box);
+        const hipError_t eg = hipGetLastError();
+        if (eg != hipSuccess) return eg;
+    }
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, rec, views, split, ip, bcnt, bcnt_all);
+                       vis_count, box, views, split, ip, bcnt, bcnt_all);
     hipError_t e = hipGetLastError();
     // Per view: block offsets, and the view's pair total; then the view bases and the total.
     if (e == hipSuccess) e = launch_scan_rows(s, bcnt, bpv, nviews, nullptr, vtot);
@@ -374,12 +400,12 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     if (e != hipSuccess) return e;
     if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           vis_count, rec, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
+                           vis_count, box, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
     } else {
         const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_dense, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv, nviews,
-                           vis_count, rec, views, split, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
+                           vis_count, box, views, split, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
     }
     return hipGetLastError();
 }
@@ -389,14 +415,15 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
 uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THREADS : 1; }
 
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                       const uint32_t *vis_count, const uint4 *rec, const ViewParams *views, EmitSplit split,
-                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
+                       const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                       EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
                        uint32_t *view_pairs, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
                        uint32_t *counters) {
     if (phase == EMIT_SECOND)
-        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, rec, views, split, ip, bcnt, bcnt_all,
-                                       vtot, view_pairs, pkey, pval, powner, pair_cap, CNT_PAIRS2, counters);
-    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, rec, views, split, ip, bcnt, bcnt_all,
+        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt,
+                                       bcnt_all, vtot, view_pairs, pkey, pval, powner, pair_cap, CNT_PAIRS2,
+                                       counters);
+    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt, bcnt_all,
                                   vtot, view_pairs, pkey, pval, powner, pair_cap, CNT_PAIRS, counters);
 }
 
