@@ -162,6 +162,8 @@ struct sp_handle {
     uint32_t *d_pixlast = nullptr; // pix_cap: tile-list position after the last contribution
     uint64_t pix_cap = 0;
     float *d_gpair = nullptr;  // NG2D x gpair_cap per-pair gradients
+    uint32_t *d_stamp = nullptr;  // gpair_cap: id of the backward that last wrote the pair's gradients
+    uint32_t stamp_id = 0;
     uint64_t gpair_cap = 0;
     float *d_g2d = nullptr;  // NG2D x g2d_cap per-record gradients
     uint64_t g2d_cap = 0;
@@ -338,10 +340,14 @@ int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
     int rc = SP_OK;
     if (pairs > h->gpair_cap) {
         dfree(h, h->d_gpair, h->gpair_cap * sp::NG2D * 4);
+        dfree(h, h->d_stamp, h->gpair_cap * 4);
         h->gpair_cap = 0;
         rc = dalloc(h, (void **)&h->d_gpair, pairs * sp::NG2D * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_stamp, pairs * 4);
         if (rc) return rc;
         h->gpair_cap = pairs;
+        h->stamp_id = 0;  // stamps restart: cleared below before the next id is used
+        HIP_TRY(sp::launch_fill(h->stream, h->d_stamp, pairs, 0u));
     }
     if (records > h->g2d_cap) {
         dfree(h, h->d_g2d, h->g2d_cap * sp::NG2D * 4);
@@ -712,6 +718,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_pixT);
     (void)hipFree(h->d_pixlast);
     (void)hipFree(h->d_gpair);
+    (void)hipFree(h->d_stamp);
     (void)hipFree(h->d_g2d);
     (void)hipFree(h->d_part);
     (void)hipFree(h->d_done);
@@ -1121,11 +1128,18 @@ int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *
     for (uint32_t v = 0; v < nv; ++v) h->h_views[v].gimg = grad_images[v];
     hipStream_t s = h->stream;
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nv * sizeof(ViewParams), hipMemcpyHostToDevice, s));
-    HIP_TRY(sp::launch_fill_f32(s, h->d_gpair, h->gpair_cap * sp::NG2D, 0.0f));
+    // Per-pair gradients are not cleared: each backward stamps the pairs it
+    // writes with a fresh id and the reduction ignores the others.
+    if (++h->stamp_id == 0) {
+        HIP_TRY(sp::launch_fill(s, h->d_stamp, h->gpair_cap, 0u));
+        h->stamp_id = 1;
+    }
     HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, h->g2d_cap * sp::NG2D, 0.0f));
     HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->d_tile_start, h->d_pval[h->pair_buf],
-                                 h->d_powner, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap));
-    HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_powner, np, h->d_g2d, h->g2d_cap, h->d_part));
+                                 h->d_powner, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
+                                 h->d_stamp, h->stamp_id));
+    HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_stamp, h->stamp_id, h->d_powner, np, h->d_g2d,
+                                  h->g2d_cap, h->d_part));
     HIP_TRY(sp::launch_project_bwd(s, dev_scene(h), h->d_views, nv, h->d_g2d, h->g2d_cap, grads->means,
                                    grads->log_scales, grads->quats, grads->opacity_logits, grads->colors,
                                    grads->sh_rest));

@@ -18,12 +18,16 @@
 //        dL/dmean2 = dL/dpw * (A dx + B dy, B dx + C dy)
 //        dL/dconic = dL/dpw * (-dx^2 / 2, -dx dy, -dy^2 / 2)
 //      with G = dL/dpixel. The nine values are summed over the tile's 256
-//      pixels in a fixed order (wave butterflies, then the four waves) and
-//      stored per pair at its emission index.
+//      pixels in a fixed order (DPP sums within a wave, then the four waves) and
+//      stored per pair at its emission index. Records whose falloff cannot
+//      reach a quadrant are skipped by its wave.
 //   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
-//      sums each (view, splat) record's pairs: segments inside a 256-pair
-//      block are summed sequentially from LDS; a segment that crosses blocks
-//      is finished by its first block, which adds the partial sums its
+//      sums each (view, splat) record's pairs. A per-pair stamp tells the
+//      pairs this backward wrote from stale ones, so nothing is cleared.
+//      Within a 256-pair block, ballots mark segment heads and written pairs;
+//      each head finds its segment's end from the head mask and adds only
+//      the written pairs, in index order. A segment that crosses blocks is
+//      finished by its first block, which adds the partial sums its
 //      successors left.
 //   3. k_project_bwd: one thread per splat chains the per-view 2D gradients
 //      through the EWA projection, the covariance (scale, rotation), the
@@ -43,9 +47,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
     const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ powner,
     const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
-    float *__restrict__ gpair, uint64_t gstride, uint32_t per_xcd) {
+    float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ stamp, uint32_t stamp_id,
+    uint32_t per_xcd) {
     __shared__ uint4 s_rec[BWD_BATCH][3];
     __shared__ uint32_t s_e[BWD_BATCH];
+    __shared__ uint32_t s_qm[BWD_BATCH];  // quadrant_mask of each staged record
     __shared__ float s_part[4][NG2D][BWD_BATCH];
     __shared__ uint32_t s_wmax[4];
     const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
@@ -91,17 +97,19 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             const uint32_t e = sval[start + lo + threadIdx.x];
             s_e[threadIdx.x] = e;
             const uint4 *r = rec + 3 * (uint64_t)powner[e];
-            s_rec[threadIdx.x][0] = r[0];
-            s_rec[threadIdx.x][1] = r[1];
-            s_rec[threadIdx.x][2] = r[2];
+            const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
+            s_rec[threadIdx.x][0] = r0;
+            s_rec[threadIdx.x][1] = r1;
+            s_rec[threadIdx.x][2] = r2;
+            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy);
         }
         for (uint32_t z = threadIdx.x; z < 4u * NG2D * BWD_BATCH; z += BLEND_THREADS) part[z] = 0.0f;
         __syncthreads();
         for (int k = (int)nb - 1; k >= 0; --k) {
             const uint32_t pos = lo + (uint32_t)k;
+            if (!((s_qm[k] >> w) & 1u)) continue;  // wave-uniform: no pixel of the quadrant blended it
             const uint4 r2 = s_rec[k][2];
             const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-            if (bx1 <= qx0 || bx0 >= qx0 + 8u || by1 <= qy0 || by0 >= qy0 + 8u) continue;  // wave-uniform
             const bool act = pos < last && px >= bx0 && px < bx1 && py >= by0 && py < by1;
             if (!__any(act)) continue;
             const uint4 r0 = s_rec[k][0];
@@ -150,11 +158,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
                     gv[4] = -0.5f * gpw * (dy * dy);
                 }
             }
-            if (__any(con)) {
+            if (__any(con)) {  // wave-uniform, so all 64 lanes take part in the DPP sums
 #pragma unroll
                 for (int c = 0; c < NG2D; ++c) {
-                    const float sum = wave_sum_f(gv[c]);
-                    if (lane_i == 0) s_part[w][c][k] = sum;
+                    const float sum = wave_total_f(gv[c]);
+                    if (lane_i == 63) s_part[w][c][k] = sum;
                 }
             }
         }
@@ -165,6 +173,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             for (int c = 0; c < NG2D; ++c)
                 gpair[c * gstride + e] = (s_part[0][c][threadIdx.x] + s_part[1][c][threadIdx.x]) +
                                          (s_part[2][c][threadIdx.x] + s_part[3][c][threadIdx.x]);
+            stamp[e] = stamp_id;  // this call wrote gpair[.., e]
         }
         __syncthreads();
         hi = lo;
@@ -176,31 +185,63 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
 // the previous block (head, slot 0) and of one continuing into the next
 // (tail, slot 1); a block covered by one segment stores its sum as head.
 __global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restrict__ gpair, uint64_t gstride,
+                                                           const uint32_t *__restrict__ stamp, uint32_t stamp_id,
                                                            const uint32_t *__restrict__ powner, uint64_t np,
                                                            float *__restrict__ g2d, uint64_t g2d_stride,
                                                            float *__restrict__ part) {
+    constexpr int NW = SR_THREADS / 64;
     __shared__ float s_v[NG2D][SR_THREADS];
     __shared__ uint32_t s_o[SR_THREADS];
+    __shared__ uint64_t s_fresh[NW], s_head[NW];
     const uint64_t base = (uint64_t)blockIdx.x * SR_THREADS;
     const uint32_t cnt = (uint32_t)min((uint64_t)SR_THREADS, np - base);
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    // Pairs past a tile's last used position were not written by this
+    // backward: they contribute zero and are never read.
+    bool fresh = false;
     if (t < cnt) {
         s_o[t] = powner[base + t];
+        fresh = stamp[base + t] == stamp_id;
+        if (fresh)
 #pragma unroll
-        for (int c = 0; c < NG2D; ++c) s_v[c][t] = gpair[c * gstride + base + t];
+            for (int c = 0; c < NG2D; ++c) s_v[c][t] = gpair[c * gstride + base + t];
     }
     __syncthreads();
-    if (t >= cnt) return;
-    const uint32_t o = s_o[t];
-    if (t > 0 && s_o[t - 1] == o) return;  // not the first element of its segment in this block
-    uint32_t u = t + 1;
-    while (u < cnt && s_o[u] == o) ++u;
+    const uint32_t o = t < cnt ? s_o[t] : 0u;
+    const bool head = t >= cnt || t == 0 || s_o[t - 1] != o;  // first element of a segment (or past the end)
+    const uint64_t fm = __ballot(fresh), hm = __ballot(head);
+    if (lane == 0) {
+        s_fresh[w] = fm;
+        s_head[w] = hm;
+    }
+    __syncthreads();
+    if (t >= cnt || !head) return;
+    // Segment [t, u): u = the next head (or cnt).
+    uint32_t u = cnt;
+    for (uint32_t ww = w; ww < NW; ++ww) {
+        uint64_t m = s_head[ww];
+        if (ww == w) m &= (lane == 63) ? 0ull : (~0ull << (lane + 1));
+        if (m) {
+            u = min(cnt, ww * 64 + (uint32_t)__builtin_ctzll(m));
+            break;
+        }
+    }
     float sum[NG2D];
 #pragma unroll
     for (int c = 0; c < NG2D; ++c) sum[c] = 0.0f;
-    for (uint32_t j = t; j < u; ++j)
+    // Add the written pairs of the segment in index order.
+    for (uint32_t ww = w; ww * 64 < u; ++ww) {
+        uint64_t m = s_fresh[ww];
+        if (ww == w) m &= ~0ull << lane;
+        const uint32_t hi = u - ww * 64;
+        if (hi < 64) m &= (1ull << hi) - 1ull;
+        while (m) {
+            const uint32_t j = ww * 64 + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1ull;
 #pragma unroll
-        for (int c = 0; c < NG2D; ++c) sum[c] += s_v[c][j];
+            for (int c = 0; c < NG2D; ++c) sum[c] += s_v[c][j];
+        }
+    }
     const bool before = t == 0 && base > 0 && powner[base - 1] == o;
     const bool after = u == cnt && base + cnt < np && powner[base + cnt] == o;
     if (!before && !after) {
@@ -493,22 +534,23 @@ void launch_pbwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uin
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                             const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
-                            uint64_t gstride) {
+                            uint64_t gstride, uint32_t *stamp, uint32_t stamp_id) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
     hipLaunchKernelGGL(k_blend_bwd, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, views, nviews, total_tiles,
-                       tile_start, sval, powner, rec, pix_T, pix_last, gpair, gstride, per_xcd);
+                       tile_start, sval, powner, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id, per_xcd);
     return hipGetLastError();
 }
 
 uint64_t seg_blocks(uint64_t np) { return (np + SR_THREADS - 1) / SR_THREADS; }
 
-hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *powner, uint64_t np,
-                             float *g2d, uint64_t g2d_stride, float *part) {
+hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
+                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, uint64_t g2d_stride,
+                             float *part) {
     if (np == 0) return hipSuccess;
     const uint64_t nb = seg_blocks(np);
-    hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(SR_THREADS), 0, s, gpair, gstride, powner, np, g2d,
-                       g2d_stride, part);
+    hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(SR_THREADS), 0, s, gpair, gstride, stamp, stamp_id,
+                       powner, np, g2d, g2d_stride, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_seg_carry, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, powner, np, (uint32_t)nb,

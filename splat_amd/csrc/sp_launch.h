@@ -86,18 +86,20 @@ hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32
 
 // ---- diff.hip: adjoint of the blend and of the projection ----
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission
-// index) of every tile's list up to its last used position; pairs after it
-// are left untouched (the caller zero-fills gpair).
+// index) of every tile's list up to its last used position, each written pair
+// stamped with stamp_id; pairs after it are left untouched.
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                             const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
-                            uint64_t gstride);
+                            uint64_t gstride, uint32_t *stamp, uint32_t stamp_id);
 // Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.
 uint64_t seg_blocks(uint64_t np);
-// g2d[c * g2d_stride + record] = sum of gpair over the record's pairs (records
-// without pairs are untouched; the caller zero-fills g2d).
-hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *powner, uint64_t np,
-                             float *g2d, uint64_t g2d_stride, float *part);
+// g2d[c * g2d_stride + record] = sum over the record's pairs of gpair where
+// stamp == stamp_id (others count as zero). Records without pairs are
+// untouched; the caller zero-fills g2d.
+hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
+                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, uint64_t g2d_stride,
+                             float *part);
 // Per-gaussian parameter gradients (sc.storage == STORE_PARAMS), summed over views.
 hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
                               const float *g2d, uint64_t g2d_stride, float *g_means, float *g_log_scales,
