@@ -101,7 +101,9 @@ def main() -> None:
     log(f"generating {args.splats} SH-3 splats")
     scene = S.random_scene(args.splats, seed=1, sh_degree=3)
     views = S.orbit_views(scene, 40, 20.0)
-    r = S.HipRenderer(scene)
+    # Timed views run without stage timing: each view is one graph replay, as a
+    # caller that does not ask for stage times gets it.
+    r = S.HipRenderer(scene, timing=False)
     out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
     bg = (0.0, 0.0, 0.0)
 
@@ -130,6 +132,14 @@ def main() -> None:
     value = args.steps / elapsed
     mean = lambda xs: float(np.mean(xs)) if xs else 0.0
 
+    # Stage breakdown: a second handle with timing events between the stages
+    # (same tuning), over the first views of the orbit.
+    log("stage timing")
+    rt = S.HipRenderer(scene)
+    rt.set_tuning(**{k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")})
+    rt.render_device(views[0], W, H, bg, out.data_ptr())
+    stats = [rt.render_device(views[i % len(views)], W, H, bg, out.data_ptr()) for i in range(min(args.steps, 8))]
+    rt.close()
     proj_ms = mean([s.project_ms for s in stats])
     sort_ms = mean([s.sort_ms for s in stats])
     blend_ms = mean([s.raster_ms for s in stats])

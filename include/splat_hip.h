@@ -94,7 +94,8 @@ typedef struct sp_options {
     int32_t storage;           /* SP_STORE_* */
     uint64_t initial_pairs;    /* initial pair capacity (0 = automatic) */
     uint64_t max_device_bytes; /* 0 = unlimited; above it allocation fails with SP_E_NOMEM */
-    int32_t timing;            /* 1 = fill the *_ms fields of sp_stats from HIP events */
+    int32_t timing;            /* 1 = fill the *_ms fields of sp_stats from HIP events; 0 = no stage
+                                  times, and a single view replays a captured graph */
     int32_t reserved;
 } sp_options;
 

@@ -98,14 +98,27 @@ double now_ms() {
 // (device-resident) view table.
 struct GraphKey {
     uint32_t nviews = 0, total_tiles = 0, tile_bits = 0, with_blend = 0;
-    uint32_t split_shift = 0, split_kmin = 0, blend_batch = 0;
+    uint32_t split_shift = 0, split_kmin = 0, blend_batch = 0, total_sbs = 0;
     uint64_t generation = 0;
     bool operator==(const GraphKey &o) const {
         return nviews == o.nviews && total_tiles == o.total_tiles && tile_bits == o.tile_bits &&
+               total_sbs == o.total_sbs &&
                with_blend == o.with_blend && split_shift == o.split_shift && split_kmin == o.split_kmin &&
                blend_batch == o.blend_batch && generation == o.generation;
     }
 };
+
+// A captured pipeline and the shape it was captured for.
+struct GraphEntry {
+    GraphKey key{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int pair_buf = 0;
+    uint64_t used = 0;  // last use (handle clock), for least-recently-used eviction
+};
+// Shapes kept captured per handle (callers alternating a few resolutions
+// replay their graphs instead of capturing again).
+constexpr int GRAPH_CACHE = 8;
 
 }  // namespace
 
@@ -125,11 +138,12 @@ struct sp_handle {
     uint4 *d_rec = nullptr;
     uint2 *d_box = nullptr;  // depth-ordered pixel boxes, view_cap x n
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
-    uint32_t *d_vstat = nullptr;  // 3 x view_cap: visible, pairs, emission view bases
+    uint32_t *d_vstat = nullptr;  // 4 x view_cap: visible, pairs, emission view bases, superblock-pair bases
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
     uint32_t *d_bcnt_all = nullptr;  // view_cap x block counts of the unsplit emission
+    uint32_t *d_bsb = nullptr;    // view_cap x block counts of superblock pairs (binned first phase)
     ViewParams *d_views = nullptr, *h_views = nullptr;
-    uint32_t *d_hist = nullptr;  // view_cap x 4 passes x 256 digits
+    uint32_t *d_rowtot = nullptr;  // radix digit-row totals: view_cap x 256 digits
     uint32_t *d_table = nullptr;  // radix per-block digit counts
     uint64_t table_words = 0;
     // Pairs of all views.
@@ -137,6 +151,10 @@ struct sp_handle {
     uint32_t *d_pkey[2] = {nullptr, nullptr}, *d_pval[2] = {nullptr, nullptr};
     uint32_t *d_tile_start = nullptr;
     uint64_t tile_cap = 0;
+    // Binned first phase: superblock list bounds, per-tile counts and row totals.
+    uint32_t *d_sb_start = nullptr, *d_tcount = nullptr, *d_trow = nullptr;
+    uint32_t *d_segc = nullptr;  // (sb_cap - 1) x BIN_SEGS x 16 segment counts
+    uint64_t sb_cap = 0, tcount_cap = 0;
     uint8_t *d_rgb = nullptr;
     uint64_t rgb_cap = 0;
     uint32_t *d_counters = nullptr;
@@ -171,11 +189,9 @@ struct sp_handle {
     uint64_t part_cap = 0;
     uint32_t fwd_views = 0, fwd_tiles = 0;  // shape of the last differentiable forward
     uint64_t fwd_pairs = 0;
-    // Captured batch graph.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    GraphKey gkey{};
-    int graph_pair_buf = 0;
+    // Captured pipelines, by shape.
+    GraphEntry graphs[GRAPH_CACHE];
+    uint64_t graph_clock = 0;
 };
 
 namespace {
@@ -206,12 +222,32 @@ void dfree(sp_handle *h, T *&p, uint64_t bytes) {
     p = nullptr;
 }
 
-void drop_graph(sp_handle *h) {
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
-    h->gexec = nullptr;
-    h->graph = nullptr;
-    h->gkey = GraphKey{};
+void drop_graph(GraphEntry &g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = GraphEntry{};
+}
+
+void drop_graphs(sp_handle *h) {
+    for (GraphEntry &g : h->graphs) drop_graph(g);
+}
+
+// The cached graph for `key`, or the entry to capture it into: an empty one,
+// one captured before a buffer moved (it can never match again), or the least
+// recently used.
+GraphEntry &graph_slot(sp_handle *h, const GraphKey &key, bool *hit) {
+    *hit = false;
+    for (GraphEntry &g : h->graphs)
+        if (g.exec && g.key == key) {
+            *hit = true;
+            return g;
+        }
+    GraphEntry *victim = &h->graphs[0];
+    for (GraphEntry &g : h->graphs) {
+        if (!g.exec || g.key.generation != h->generation) return g;
+        if (g.used < victim->used) victim = &g;
+    }
+    return *victim;
 }
 
 // Radix count table large enough for the depth sort of view_cap views and the
@@ -266,11 +302,12 @@ int ensure_views(sp_handle *h, uint32_t views) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
     }
-    dfree(h, h->d_vstat, old * 12);
+    dfree(h, h->d_vstat, old * 16);
     dfree(h, h->d_bcnt, old * blocks * 4);
     dfree(h, h->d_bcnt_all, old * blocks * 4);
+    dfree(h, h->d_bsb, old * blocks * 4);
     dfree(h, h->d_views, old * sizeof(ViewParams));
-    dfree(h, h->d_hist, old * 4 * sp::RADIX_BINS * 4);
+    dfree(h, h->d_rowtot, old * sp::RADIX_BINS * 4);
     if (h->h_views) (void)hipHostFree(h->h_views);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     h->h_views = nullptr;
@@ -283,11 +320,12 @@ int ensure_views(sp_handle *h, uint32_t views) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
     }
-    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 12);
+    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 16);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt_all, v * blocks * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_bsb, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_views, v * sizeof(ViewParams));
-    if (!rc) rc = dalloc(h, (void **)&h->d_hist, v * 4 * sp::RADIX_BINS * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_rowtot, v * sp::RADIX_BINS * 4);
     if (rc) return rc;
     if (hipHostMalloc((void **)&h->h_views, v * sizeof(ViewParams), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&h->h_counters, (sp::CNT_SLOTS + 2 * v) * 4, hipHostMallocDefault) != hipSuccess)
@@ -303,6 +341,32 @@ int ensure_tiles(sp_handle *h, uint64_t tiles) {
     const int rc = dalloc(h, (void **)&h->d_tile_start, (tiles + 1) * 4);
     if (rc) return rc;
     h->tile_cap = tiles + 1;
+    return SP_OK;
+}
+
+// Superblock bounds and tile counts of the binned first phase.
+int ensure_bin(sp_handle *h, uint64_t tiles, uint64_t sbs) {
+    if (sbs + 1 > h->sb_cap) {
+        const uint64_t seg_words = (uint64_t)sp::BIN_SEGS * sp::SB * sp::SB;
+        dfree(h, h->d_sb_start, h->sb_cap * 4);
+        dfree(h, h->d_segc, (h->sb_cap ? h->sb_cap - 1 : 0) * seg_words * 4);
+        h->sb_cap = 0;
+        int rc = dalloc(h, (void **)&h->d_sb_start, (sbs + 1) * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_segc, sbs * seg_words * 4);
+        if (rc) return rc;
+        h->sb_cap = sbs + 1;
+    }
+    const uint64_t rows = sp::bin_scan_rows((uint32_t)tiles);
+    if (rows * sp::BIN_SCAN_ROW > h->tcount_cap) {
+        const uint64_t old = h->tcount_cap;
+        dfree(h, h->d_tcount, old * 4);
+        dfree(h, h->d_trow, old / sp::BIN_SCAN_ROW * 4);
+        h->tcount_cap = 0;
+        int rc = dalloc(h, (void **)&h->d_tcount, rows * sp::BIN_SCAN_ROW * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_trow, rows * 4);
+        if (rc) return rc;
+        h->tcount_cap = rows * sp::BIN_SCAN_ROW;
+    }
     return SP_OK;
 }
 
@@ -403,12 +467,15 @@ int setup(sp_handle *h) {
 }
 
 // Enqueue the whole pipeline for the first `nviews` entries of h_views. The
-// sorted pairs of the (last) tile sort end in ping-pong buffer *pair_buf.
+// tile lists of the (last) phase end in ping-pong buffer *pair_buf. The first
+// phase is binned (superblock pairs sorted, tile lists written per
+// superblock) except on the differentiable path, whose pairs carry emission
+// indices and go through the tile sort.
 // BL_RGB with a split runs the emission, tile sort, ranges and blend twice:
 // the first phase for each view's nearest splats over all tiles, the second
 // for the rest over the tiles the first blend left unfinished.
-int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, int mode, bool events,
-            int *pair_buf) {
+int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_sbs, int tile_bits, int mode,
+            bool events, int *pair_buf) {
     const sp::DevScene sc = dev_scene(h);
     hipStream_t s = h->stream;
     uint32_t *vis = h->d_vstat;
@@ -424,17 +491,36 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_dkey[1], h->d_bcnt, vis, h->d_dkey[0],
                                h->d_dval[0], nullptr, nullptr, nullptr));
     int di = 0;
-    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_hist, h->d_table, &di));
+    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di));
     h->depth_buf = di;
+    uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
+    const bool binned = mode != BL_DIFF;
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box, h->d_views,
-                            es, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0], h->d_pval[0],
-                            powner, h->pair_cap, h->d_counters));
+                            es, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, binned ? h->d_bsb : nullptr,
+                            binned ? vsb : nullptr, h->d_pkey[0], h->d_pval[0], powner, h->pair_cap,
+                            h->d_counters));
     HIP_TRY(mark(1));
     int pi = 0;
-    HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
-                                  tile_bits, h->d_hist, h->d_table, &pi));
-    HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS, h->pair_cap, total_tiles,
-                              h->d_tile_start));
+    if (binned) {
+        int si = 0;
+        const int sb_bits = (int)bits_for(total_sbs ? total_sbs - 1 : 0);
+        HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_SBPAIRS, 1, 0, h->pair_cap,
+                                      sb_bits, h->d_rowtot, h->d_table, &si));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[si], h->d_counters + sp::CNT_SBPAIRS, h->pair_cap, total_sbs,
+                                  h->d_sb_start));
+        pi = 1 - si;
+        // Tile ids are kept only when the lists are read back (no blend).
+        HIP_TRY(sp::launch_bin_lists(s, h->d_views, nviews, total_sbs, total_tiles, h->d_sb_start, h->d_pval[si],
+                                     h->d_box, h->d_dval[di], h->n, h->d_segc, h->d_tcount, h->d_trow,
+                                     h->d_tile_start,
+                                     mode == BL_NONE ? h->d_pkey[pi] : nullptr, h->d_pval[pi], h->pair_cap,
+                                     h->d_counters));
+    } else {
+        HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
+                                      tile_bits, h->d_rowtot, h->d_table, &pi));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS, h->pair_cap, total_tiles,
+                                  h->d_tile_start));
+    }
     HIP_TRY(mark(2));
     if (mode != BL_NONE) {
         const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
@@ -446,13 +532,13 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, int tile_bits, 
     if (split) {
         HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip));
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box,
-                                h->d_views, es, h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, h->d_pkey[0],
-                                h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
+                                h->d_views, es, h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
+                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));
     if (split) {
         HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS2, 1, 0, h->pair_cap,
-                                      tile_bits, h->d_hist, h->d_table, &pi));
+                                      tile_bits, h->d_rowtot, h->d_table, &pi));
         HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS2, h->pair_cap, total_tiles,
                                   h->d_tile_start));
     }
@@ -482,12 +568,14 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         return fail(SP_E_INVALID, "views x gaussians exceeds 2^32 records");
     int rc = ensure_views(h, nviews);
     if (rc) return rc;
-    uint64_t total = 0;
+    uint64_t total = 0, total_sb = 0;
     for (uint32_t v = 0; v < nviews; ++v) {
         ViewParams &p = h->h_views[v];
         std::memset(&p, 0, sizeof p);
         p.ctx = make_ctx(views[v], wh[2 * v], wh[2 * v + 1], bg);
         p.tile_base = (uint32_t)total;
+        p.sb_base = (uint32_t)total_sb;
+        total_sb += (uint64_t)((p.ctx.tiles_x + sp::SB - 1) / sp::SB) * ((p.ctx.tiles_y + sp::SB - 1) / sp::SB);
         if (outs && mode == BL_RGB) p.out = (uint8_t *)outs[v];
         if (outs && mode == BL_DIFF) p.fimg = (float *)outs[v];
         total += (uint64_t)p.ctx.tiles_x * p.ctx.tiles_y;
@@ -496,6 +584,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     const uint32_t total_tiles = (uint32_t)total;
     rc = ensure_tiles(h, total_tiles);
     if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
+    if (!rc && mode != BL_DIFF) rc = ensure_bin(h, total_tiles, total_sb);
     if (!rc && mode == BL_RGB && h->split_shift > 0) rc = ensure_split(h, total_tiles);
     if (rc) return rc;
     const int tile_bits = (int)bits_for(total_tiles ? total_tiles - 1 : 0);
@@ -511,11 +600,15 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             key.split_shift = h->split_shift;
             key.split_kmin = h->split_kmin;
             key.blend_batch = (uint32_t)h->blend_batch;
+            key.total_sbs = (uint32_t)total_sb;
             key.generation = h->generation;
-            if (!h->gexec || !(key == h->gkey)) {
-                drop_graph(h);
+            bool hit = false;
+            GraphEntry &ge = graph_slot(h, key, &hit);
+            if (!hit) {
+                drop_graph(ge);
                 HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                rc = enqueue(h, nviews, total_tiles, tile_bits, mode, false, &h->graph_pair_buf);
+                int gpb = 0;
+                rc = enqueue(h, nviews, total_tiles, (uint32_t)total_sb, tile_bits, mode, false, &gpb);
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(h->stream, &g);
                 if (rc) {
@@ -523,14 +616,16 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
                     return rc;
                 }
                 if (ce != hipSuccess) return fail(SP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
-                h->graph = g;
-                HIP_TRY(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
-                h->gkey = key;
+                ge.graph = g;
+                HIP_TRY(hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
+                ge.key = key;
+                ge.pair_buf = gpb;
             }
-            HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
-            pb = h->graph_pair_buf;
+            ge.used = ++h->graph_clock;
+            HIP_TRY(hipGraphLaunch(ge.exec, h->stream));
+            pb = ge.pair_buf;
         } else {
-            rc = enqueue(h, nviews, total_tiles, tile_bits, mode, events, &pb);
+            rc = enqueue(h, nviews, total_tiles, (uint32_t)total_sb, tile_bits, mode, events, &pb);
             if (rc) return rc;
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
@@ -587,7 +682,9 @@ int render_one(sp_handle *h, const sp_view *view, uint32_t width, uint32_t heigh
                uint8_t *d_out, sp_stats *stats) {
     const uint32_t wh[2] = {width, height};
     void *outs[1] = {d_out};
-    return run_views(h, 1, view, wh, bg, outs, d_out != nullptr ? BL_RGB : BL_NONE, false, stats);
+    // Without stage timing the view replays a captured graph (one launch per
+    // view); with it, the stages are enqueued with timing events between them.
+    return run_views(h, 1, view, wh, bg, outs, d_out != nullptr ? BL_RGB : BL_NONE, !h->opt.timing, stats);
 }
 
 // Upload the scene in the handle's storage format.
@@ -694,7 +791,7 @@ void sp_destroy(sp_handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    drop_graph(h);
+    drop_graphs(h);
     (void)hipFree(h->d_soa);
     (void)hipFree(h->d_soa16);
     (void)hipFree(h->d_rec);
@@ -711,9 +808,14 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_views);
     (void)hipFree(h->d_table);
     (void)hipFree(h->d_tile_start);
+    (void)hipFree(h->d_sb_start);
+    (void)hipFree(h->d_segc);
+    (void)hipFree(h->d_tcount);
+    (void)hipFree(h->d_trow);
+    (void)hipFree(h->d_bsb);
     (void)hipFree(h->d_rgb);
     (void)hipFree(h->d_counters);
-    (void)hipFree(h->d_hist);
+    (void)hipFree(h->d_rowtot);
     (void)hipFree(h->d_powner);
     (void)hipFree(h->d_pixT);
     (void)hipFree(h->d_pixlast);

@@ -53,26 +53,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict
     if (row_total && threadIdx.x == 0) row_total[blockIdx.x] = carry - base;
 }
 
-// Radix digit rows: row r = (segment g, digit d) of `row_len` per-tile counts,
-// of which the first ceil(min(seg_counts[g], n_cap) / RADIX_TILE) are live;
-// the row's base is the exclusive prefix of the segment's digit histogram
-// hist[g * hist_seg_stride + 0..255] below d.
+// Radix digit rows: row r = (segment g, digit d) of `row_len` per-column
+// counts, of which the first radix_columns(min(seg_counts[g], n_cap),
+// radix_tile_keys(n_cap)) are live.
+// Each live prefix becomes its in-row exclusive scan and row_total[r] its sum
+// (the sort's downsweep turns the 256 row totals of a segment into digit
+// bases itself, so no separate histogram pass is needed).
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
-                                                                  const uint32_t *__restrict__ hist,
-                                                                  uint32_t hist_seg_stride,
                                                                   const uint32_t *__restrict__ seg_counts,
-                                                                  uint64_t n_cap) {
-    __shared__ uint32_t s_base;
-    const uint32_t g = blockIdx.x / RADIX_BINS, d = blockIdx.x % RADIX_BINS;
-    uint32_t tot;
-    const uint32_t pre = block_exclusive_scan<SCAN_THREADS>(hist[(uint64_t)g * hist_seg_stride + threadIdx.x], &tot);
-    if (threadIdx.x == d) s_base = pre;
-    __syncthreads();
+                                                                  uint64_t n_cap, uint32_t *__restrict__ row_total) {
+    const uint32_t g = blockIdx.x / RADIX_BINS;
     const uint64_t n = min((uint64_t)seg_counts[g], n_cap);
-    const uint64_t live = min(row_len, (n + RADIX_TILE - 1) / RADIX_TILE);
+    const uint64_t live = min(row_len, (uint64_t)radix_columns(n, radix_tile_keys(n_cap)));
     uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
-    uint32_t carry = s_base;
+    uint32_t carry = 0;
     for (uint64_t off = 0; off < live; off += SCAN_CHUNK) carry += scan_chunk(row, off, live, carry, true);
+    if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(256) void k_fill(uint32_t *__restrict__ p, uint64_t n, uint32_t value) {
@@ -87,8 +83,6 @@ __global__ __launch_bounds__(256) void k_fill_f32(float *__restrict__ p, uint64_
 
 }  // namespace
 
-static_assert(RADIX_BINS == SCAN_THREADS, "digit-row bases use one thread per digit");
-
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total) {
     if (rows == 0) return hipSuccess;
@@ -97,11 +91,10 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 }
 
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *hist, uint32_t hist_seg_stride, const uint32_t *seg_counts,
-                                  uint64_t n_cap) {
+                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t *row_total) {
     if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len, hist,
-                       hist_seg_stride, seg_counts, n_cap);
+    hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len,
+                       seg_counts, n_cap, row_total);
     return hipGetLastError();
 }
 

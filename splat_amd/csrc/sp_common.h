@@ -13,6 +13,10 @@ constexpr float ALPHA_MIN = 1.0f / 255.0f;
 constexpr float ALPHA_MAX = 0.99f;
 constexpr float T_MIN = 1e-4f;
 constexpr uint32_t TILE = 16;
+// Superblock: SB x SB tiles, the bin of the first emission phase (tiles.hip).
+constexpr uint32_t SB = 4;
+constexpr uint32_t BIN_SCAN_ROW = 2048;  // tiles per row of the binned tile-count scan
+constexpr uint32_t BIN_SEGS = 16;        // workgroups per superblock list in the binned first phase
 
 // Workgroup shapes.
 constexpr int PROJECT_THREADS = 256;  // one splat per thread
@@ -20,8 +24,15 @@ constexpr int PROJECT_VCHUNK = 8;     // views projected per pass over a splat h
 constexpr int EMIT_THREADS = 256;     // one depth-ordered splat per thread
 constexpr int BLEND_THREADS = 256;    // one pixel of a 16x16 tile per thread
 constexpr int RADIX_THREADS = 256;    // one digit per thread in the lookback
-constexpr int RADIX_ITEMS = 16;       // keys per thread per onesweep tile
-constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ITEMS;  // 4096 keys
+// Keys per thread of a radix tile: 16 (4096-key tiles) for large sorts, 4
+// (1024-key tiles) for sorts of up to RADIX_SMALL_MAX keys, whose 4096-key
+// tiles would leave most CUs idle.
+constexpr int RADIX_ITEMS_LARGE = 16;
+constexpr int RADIX_ITEMS_SMALL = 4;
+constexpr uint64_t RADIX_SMALL_MAX = 16ull << 20;
+__host__ __device__ constexpr uint32_t radix_tile_keys(uint64_t n_cap) {
+    return (uint32_t)(RADIX_THREADS * (n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
+}
 constexpr int RADIX_BINS = 256;       // 8-bit digits
 constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
@@ -31,6 +42,7 @@ enum Counter : int {
     CNT_OVERFLOW = 2,  // 1 when an emission phase exceeded the pair capacity
     CNT_STAGED = 3,    // pair records the blends gathered (early termination skips the rest)
     CNT_PAIRS2 = 4,    // pairs of the second emission phase
+    CNT_SBPAIRS = 5,   // (superblock, splat) pairs of the binned first phase
     CNT_SLOTS = 16
 };
 
@@ -60,13 +72,15 @@ struct ViewCtx {
 };
 
 // One view of a batch, in device memory: its context, the first global tile id
-// of its tiles (views' tiles are numbered consecutively) and its RGB8 output.
+// of its tiles (views' tiles are numbered consecutively), the first global
+// superblock id of its ceil(tiles_x / SB) x ceil(tiles_y / SB) superblocks
+// (numbered the same way) and its RGB8 output.
 // The differentiable path writes a float image (fimg, w*h*3, C + T*bg before
 // quantisation) instead and reads the loss gradient of that image (gimg).
 struct ViewParams {
     ViewCtx ctx;
     uint32_t tile_base;
-    uint32_t pad0;
+    uint32_t sb_base;
     uint8_t *out;
     float *fimg;
     const float *gimg;

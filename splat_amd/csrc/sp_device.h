@@ -49,6 +49,20 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     return pre + inc - v;
 }
 
+// Radix sort table columns. The keys of a segment form tiles of tile_keys
+// keys; a table column (one digit count per column, one workgroup task)
+// covers `group` consecutive tiles: 1 for small sorts, up to 8 when there are
+// enough tiles to keep 2048 columns, which cuts the scattered table traffic of
+// large sorts.
+__device__ __forceinline__ uint32_t radix_tiles(uint64_t n, uint32_t tile_keys) {
+    return (uint32_t)((n + tile_keys - 1) / tile_keys);
+}
+__device__ __forceinline__ uint32_t radix_group(uint32_t tiles) { return min(8u, max(1u, tiles / 2048u)); }
+__device__ __forceinline__ uint32_t radix_columns(uint64_t n, uint32_t tile_keys) {
+    const uint32_t t = radix_tiles(n, tile_keys), g = radix_group(t);
+    return (t + g - 1) / g;
+}
+
 // The view of a batch that owns global tile gt: the last view whose
 // tile_base <= gt (bases ascend with the view index).
 __device__ __forceinline__ uint32_t view_of_tile(const ViewParams *views, uint32_t nviews, uint32_t gt) {

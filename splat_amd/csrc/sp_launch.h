@@ -14,12 +14,11 @@ namespace sp {
 // row); row_base[r] (optional) is added; row_total[r] (optional) gets the sum.
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total);
-// Radix tables: rows (segment g, digit d), based at the exclusive prefix of
-// hist[g * hist_seg_stride + 0..255] below d; only the live prefix of each row
-// (ceil(min(seg_counts[g], n_cap) / RADIX_TILE) entries) is scanned.
+// Radix tables: rows (segment g, digit d); only the live prefix of each row
+// (radix_columns(min(seg_counts[g], n_cap), radix_tile_keys(n_cap)) entries) is scanned, in place
+// and from 0; row_total[g * 256 + d] receives the row's sum.
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *hist, uint32_t hist_seg_stride, const uint32_t *seg_counts,
-                                  uint64_t n_cap);
+                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t *row_total);
 // p[0..n) = value (a kernel, so it may be captured in a graph).
 hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 
@@ -42,9 +41,9 @@ uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap);
 // `nseg` independent segments: segment g holds min(seg_counts[g], n_cap) keys
 // at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
 // between keys[0]/keys[1]; *out_index names the buffer with the result.
-// hist holds nseg * 4 * 256 u32, table radix_table_words(nseg, n_cap) u32.
+// row_total holds nseg * 256 u32, table radix_table_words(nseg, n_cap) u32.
 hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
-                             uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *hist,
+                             uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
                              uint32_t *table, int *out_index);
 
 // ---- tiles.hip: (b) emission, (d) ranges ----
@@ -59,11 +58,26 @@ uint32_t emit_blocks(uint32_t n);
 // also writes each view's unsplit pair count to view_pairs[v]. With powner
 // set, pval[e] = e and powner[e] = record index instead.
 // box: n entries per view, written by FIRST (the depth-ordered pixel boxes) and read by SECOND.
+// Binned FIRST (bsb / vsb given, shaped like bcnt / vtot): the pairs are
+// (global superblock id, depth slot v * n + j) instead, counted in
+// counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts the tile pairs.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
-                       uint32_t *view_pairs, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
-                       uint32_t *counters);
+                       uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
+                       uint32_t *powner, uint64_t pair_cap, uint32_t *counters);
+// Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
+// entries, trow bin_scan_rows.
+uint32_t bin_scan_rows(uint32_t total_tiles);
+// Binned FIRST lists: from the superblock-sorted depth slots sslot (ranges
+// sb_start[0..total_sbs]), the tile lists in tile order (pval = record index,
+// pkey = tile id when pkey is given) and tile_start[0..total_tiles]. segc:
+// total_sbs * BIN_SEGS * SB * SB scratch counts.
+hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
+                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *sslot, const uint2 *box,
+                            const uint32_t *dval, uint32_t n_splats, uint32_t *segc, uint32_t *tcount,
+                            uint32_t *trow, uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
+                            const uint32_t *counters);
 // Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every view.
 hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
                              uint32_t *ip);

@@ -72,6 +72,23 @@ __device__ __forceinline__ Rect splat_rect(const uint32_t *__restrict__ dval, co
     return r;
 }
 
+// Superblock rectangle of a tile rectangle: the superblocks (SB x SB tiles)
+// it overlaps, as count, first global superblock id and width.
+struct SbRect {
+    uint32_t cnt, org, rw;
+};
+__device__ __forceinline__ SbRect sb_rect(const Rect &r, uint32_t sbx, uint32_t sb_base) {
+    SbRect q{0u, 0u, 1u};
+    if (r.cnt) {
+        const uint32_t sx0 = r.tx0 / SB, sy0 = r.ty0 / SB;
+        const uint32_t sx1 = (r.tx1 + SB - 1) / SB, sy1 = (r.ty1 + SB - 1) / SB;
+        q.rw = sx1 - sx0;
+        q.cnt = q.rw * (sy1 - sy0);
+        q.org = sb_base + sy0 * sbx + sx0;
+    }
+    return q;
+}
+
 // Number of depth-ordered splats of a view the FIRST phase emits.
 __device__ __forceinline__ uint32_t first_count(uint32_t vis, EmitSplit sp) {
     const uint32_t f = (uint32_t)(((uint64_t)vis + (1ull << sp.shift) - 1) >> sp.shift);
@@ -110,7 +127,8 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
                                                              const ViewParams *__restrict__ views, EmitSplit split,
                                                              const uint32_t *__restrict__ ip,
                                                              uint32_t *__restrict__ bcnt,
-                                                             uint32_t *__restrict__ bcnt_all) {
+                                                             uint32_t *__restrict__ bcnt_all,
+                                                             uint32_t *__restrict__ bsb) {
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);  // dval holds n_splats entries per view
@@ -125,6 +143,12 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
         uint32_t all;
         block_exclusive_scan<EMIT_THREADS>(r.cnt, &all);
         if (threadIdx.x == 0) bcnt_all[blockIdx.x] = all;
+        if (bsb) {  // binned first phase: superblock pairs of the same splats
+            const SbRect q = sb_rect(r, (tiles_x + SB - 1) / SB, views[v].sb_base);
+            uint32_t sbt;
+            block_exclusive_scan<EMIT_THREADS>(j < k1 ? q.cnt : 0u, &sbt);
+            if (threadIdx.x == 0) bsb[blockIdx.x] = sbt;
+        }
     }
 }
 
@@ -154,6 +178,11 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *_
 // cover, and moves on to the next block until the chunk is done. The nearest
 // splats of a view cover thousands of tiles each, so splat-major work would
 // leave one workgroup with most of the pairs.
+//
+// BIN: the binned first phase expands each splat's superblock rectangle
+// instead; the pair carries the splat's depth slot (v * n + j) rather than its
+// record, and the pair count comes from counters[CNT_SBPAIRS].
+template <bool BIN>
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                               uint32_t bpv, uint32_t nviews,
                                                               const uint32_t *__restrict__ vis_count,
@@ -169,9 +198,10 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
     __shared__ uint32_t s_rw[EMIT_THREADS];   // rectangle width in tiles
     __shared__ uint32_t s_id[EMIT_THREADS];   // record index (view * n + scene index)
     __shared__ uint32_t s_v, s_blk;
-    const uint64_t np = counters[CNT_PAIRS];
+    const uint64_t np = counters[BIN ? CNT_SBPAIRS : CNT_PAIRS];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[CNT_OVERFLOW] = np > pair_cap ? 1u : 0u;
+        // The tile pairs (which a binned list holds too) set the capacity need.
+        counters[CNT_OVERFLOW] = counters[CNT_PAIRS] > pair_cap ? 1u : 0u;
         counters[CNT_STAGED] = 0u;  // the blends of this frame accumulate into it
     }
     const uint64_t nw = min(np, pair_cap);
@@ -211,13 +241,20 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
             const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
             const uint32_t j = blk * EMIT_THREADS + threadIdx.x;
             const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
+            const uint32_t gw = BIN ? (tiles_x + SB - 1) / SB : tiles_x;  // grid width of the emitted cells
+            uint32_t cnt = phase_count<EMIT_FIRST>(r, j, k1, nullptr, tile_base, tiles_x), org = r.org, rw = r.rw;
+            if (BIN) {
+                const SbRect q = sb_rect(r, gw, views[v].sb_base);
+                cnt = j < k1 ? q.cnt : 0u;
+                org = q.org;
+                rw = q.rw;
+            }
             uint32_t total;
-            const uint32_t lscan =
-                block_exclusive_scan<EMIT_THREADS>(phase_count<EMIT_FIRST>(r, j, k1, nullptr, tile_base, tiles_x), &total);
+            const uint32_t lscan = block_exclusive_scan<EMIT_THREADS>(cnt, &total);
             s_pre[threadIdx.x] = lscan;
-            s_org[threadIdx.x] = r.org;
-            s_rw[threadIdx.x] = r.rw;
-            s_id[threadIdx.x] = r.id;
+            s_org[threadIdx.x] = org;
+            s_rw[threadIdx.x] = rw;
+            s_id[threadIdx.x] = BIN ? (uint32_t)(vbase + j) : r.id;
             __syncthreads();
             const uint64_t bbase = (uint64_t)vbase_pairs[v] + boff[(uint64_t)v * bpv + blk];
             const uint64_t hi = min(c1, bbase + total);
@@ -235,9 +272,9 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
                 const uint32_t loc = lq - s_pre[lo];
                 const uint32_t rwl = s_rw[lo];
                 const uint32_t ry = loc / rwl, rx = loc - ry * rwl;
-                const uint32_t tile = s_org[lo] + ry * tiles_x + rx;
+                const uint32_t tile = s_org[lo] + ry * gw + rx;
                 pkey[qq] = tile;
-                if (powner) {
+                if (!BIN && powner) {
                     // Differentiable path: the pair carries its emission index; the
                     // record index stays behind in emission order.
                     pval[qq] = (uint32_t)qq;
@@ -352,26 +389,267 @@ __global__ __launch_bounds__(256) void k_row_prefix(const ViewParams *__restrict
     }
 }
 
+// ---- Binned first phase ----
+// The pairs of the first phase are not sorted: each splat emits one pair per
+// superblock (SB x SB tiles) its tile rectangle overlaps, those few pairs are
+// sorted by superblock (stable, so each superblock's list stays in depth
+// order), and one workgroup per superblock then writes the tile lists of its
+// 16 tiles: each 256-entry chunk of its list gives every entry a 16-bit mask
+// of the tiles it covers, one ballot per tile ranks the entries in list
+// order, and the entries of one tile land at consecutive addresses. A count
+// pass and a scan over the tiles place every list first, so the result is the
+// tile-major (tile, depth, index) order of a full tile sort, with tile_start,
+// written without moving the tile pairs through sort passes.
+
+// The view of a batch that owns global superblock sb (sb_base ascends).
+__device__ __forceinline__ uint32_t view_of_sb(const ViewParams *views, uint32_t nviews, uint32_t sb) {
+    uint32_t lo = 0, hi = nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].sb_base <= sb)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Tiles of superblock origin (bx, by) covered by a splat's pixel box, as bits
+// (row * SB + column); the tile rectangle is splat_rect's.
+__device__ __forceinline__ uint32_t sb_mask(uint2 b, uint32_t bx, uint32_t by) {
+    const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
+    if (!(x0 < x1 && y0 < y1)) return 0u;
+    const int i0 = max((int)(x0 / TILE) - (int)bx, 0), i1 = min((int)((x1 + TILE - 1) / TILE) - (int)bx, (int)SB);
+    const int j0 = max((int)(y0 / TILE) - (int)by, 0), j1 = min((int)((y1 + TILE - 1) / TILE) - (int)by, (int)SB);
+    if (i0 >= i1 || j0 >= j1) return 0u;
+    const uint32_t row = ((1u << i1) - 1u) & ~((1u << i0) - 1u);
+    uint32_t m = 0;
+    for (int jj = j0; jj < j1; ++jj) m |= row << (SB * jj);
+    return m;
+}
+
+struct SbGeom {
+    uint32_t v, bx, by, tiles_x, tiles_y, tile_base;
+};
+__device__ __forceinline__ SbGeom sb_geom(const ViewParams *views, uint32_t nviews, uint32_t sb) {
+    SbGeom g;
+    g.v = view_of_sb(views, nviews, sb);
+    const ViewParams &vp = views[g.v];
+    g.tiles_x = vp.ctx.tiles_x;
+    g.tiles_y = vp.ctx.tiles_y;
+    g.tile_base = vp.tile_base;
+    const uint32_t sbx = (g.tiles_x + SB - 1) / SB, local = sb - vp.sb_base;
+    g.bx = (local % sbx) * SB;
+    g.by = (local / sbx) * SB;
+    return g;
+}
+
+// A superblock's list is cut into BIN_SEGS segments (each a whole number of
+// 256-entry chunks, at least one), one workgroup per (superblock, segment),
+// so that long lists (the centre of a view) spread over many CUs.
+__device__ __forceinline__ void bin_segment(uint32_t e0, uint32_t e1, uint32_t y, uint32_t &s0, uint32_t &s1) {
+    const uint32_t len = e1 - e0;
+    const uint32_t per = max(256u, (((len + BIN_SEGS - 1) / BIN_SEGS) + 255u) & ~255u);
+    s0 = min(e1, e0 + y * per);
+    s1 = min(e1, s0 + per);
+}
+
+// segc[(sb * BIN_SEGS + y) * 16 + b] = entries of segment y of superblock sb
+// that cover its tile b.
+__global__ __launch_bounds__(256) void k_bin_count(const ViewParams *__restrict__ views, uint32_t nviews,
+                                                   const uint32_t *__restrict__ sb_start,
+                                                   const uint32_t *__restrict__ sslot, const uint2 *__restrict__ box,
+                                                   uint32_t *__restrict__ segc) {
+    __shared__ uint32_t s_c[4][SB * SB];
+    const uint32_t sb = blockIdx.x, y = blockIdx.y, w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t s0, s1;
+    bin_segment(sb_start[sb], sb_start[sb + 1], y, s0, s1);
+    uint32_t mine = 0;  // lane b < 16: this wave's entries covering tile b
+    if (s0 < s1) {
+        const SbGeom g = sb_geom(views, nviews, sb);
+        for (uint32_t c = s0; c < s1; c += 256) {  // block-uniform
+            const uint32_t e = c + threadIdx.x;
+            const uint32_t m = e < s1 ? sb_mask(box[sslot[e]], g.bx, g.by) : 0u;
+#pragma unroll
+            for (uint32_t b = 0; b < SB * SB; ++b) {
+                const uint32_t k = (uint32_t)__popcll(__ballot((m >> b) & 1u));
+                if (lane == b) mine += k;
+            }
+        }
+    }
+    if (lane < SB * SB) s_c[w][lane] = mine;
+    __syncthreads();
+    if (threadIdx.x < SB * SB)
+        segc[((uint64_t)sb * BIN_SEGS + y) * (SB * SB) + threadIdx.x] =
+            s_c[0][threadIdx.x] + s_c[1][threadIdx.x] + s_c[2][threadIdx.x] + s_c[3][threadIdx.x];
+}
+
+// One thread per (superblock, tile): segc becomes the exclusive prefix over
+// the superblock's segments, tcount[gt] the tile's list length; the padding
+// tcount[total_tiles, tcount_len) is cleared for the row scan.
+__global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict__ views, uint32_t nviews,
+                                                    uint32_t total_sbs, uint32_t *__restrict__ segc,
+                                                    uint32_t *__restrict__ tcount, uint32_t total_tiles,
+                                                    uint32_t tcount_len) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < (uint64_t)total_sbs * (SB * SB)) {
+        const uint32_t sb = (uint32_t)(g / (SB * SB)), b = (uint32_t)(g % (SB * SB));
+        uint32_t *c = segc + (uint64_t)sb * BIN_SEGS * (SB * SB) + b;
+        uint32_t acc = 0;
+        for (uint32_t y = 0; y < BIN_SEGS; ++y) {
+            const uint32_t k = c[y * (SB * SB)];
+            c[y * (SB * SB)] = acc;
+            acc += k;
+        }
+        const SbGeom q = sb_geom(views, nviews, sb);
+        const uint32_t tx = q.bx + b % SB, ty = q.by + b / SB;
+        if (tx < q.tiles_x && ty < q.tiles_y) tcount[q.tile_base + ty * q.tiles_x + tx] = acc;
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = total_tiles + g; t < tcount_len; t += stride) tcount[t] = 0;
+}
+
+// Writes tile_start for the superblock's tiles (segment 0; clamped to the
+// capacity: an over-capacity frame is rendered again with larger buffers) and
+// the segment's entries into the tile lists: pval = record index, pkey (when
+// given) = tile id. tscan: in-row exclusive prefix of tcount (rows of
+// BIN_SCAN_ROW tiles); trow: exclusive prefix of the row totals; segc: the
+// segment offsets within each tile's list.
+__global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict__ views, uint32_t nviews,
+                                                   uint32_t n_splats, const uint32_t *__restrict__ sb_start,
+                                                   const uint32_t *__restrict__ sslot, const uint2 *__restrict__ box,
+                                                   const uint32_t *__restrict__ dval,
+                                                   const uint32_t *__restrict__ segc,
+                                                   const uint32_t *__restrict__ tscan,
+                                                   const uint32_t *__restrict__ trow, uint32_t total_tiles,
+                                                   uint32_t *__restrict__ tile_start, uint32_t *__restrict__ pkey,
+                                                   uint32_t *__restrict__ pval, uint64_t pair_cap,
+                                                   const uint32_t *__restrict__ counters) {
+    __shared__ uint32_t s_next[SB * SB];  // next list position of each tile
+    __shared__ uint32_t s_wc[4][SB * SB];
+    const uint32_t sb = blockIdx.x, y = blockIdx.y, w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t s0, s1;
+    bin_segment(sb_start[sb], sb_start[sb + 1], y, s0, s1);
+    if (sb == 0 && y == 0 && threadIdx.x == 0)
+        tile_start[total_tiles] = (uint32_t)min((uint64_t)counters[CNT_PAIRS], pair_cap);
+    if (s0 >= s1 && y > 0) return;  // block-uniform; segment 0 always writes the tile starts
+    const SbGeom g = sb_geom(views, nviews, sb);
+    const uint64_t vbase = (uint64_t)g.v * n_splats;
+    if (threadIdx.x < SB * SB) {
+        const uint32_t tx = g.bx + (threadIdx.x % SB), ty = g.by + (threadIdx.x / SB);
+        uint32_t start = 0;
+        if (tx < g.tiles_x && ty < g.tiles_y) {
+            const uint32_t gt = g.tile_base + ty * g.tiles_x + tx;
+            start = trow[gt / BIN_SCAN_ROW] + tscan[gt];
+            if (y == 0) tile_start[gt] = (uint32_t)min((uint64_t)start, pair_cap);
+        }
+        s_next[threadIdx.x] = start + segc[((uint64_t)sb * BIN_SEGS + y) * (SB * SB) + threadIdx.x];
+    }
+    __syncthreads();
+    for (uint32_t c = s0; c < s1; c += 256) {  // block-uniform
+        const uint32_t e = c + threadIdx.x;
+        uint32_t m = 0, rid = 0;
+        if (e < s1) {
+            const uint32_t slot = sslot[e];
+            m = sb_mask(box[slot], g.bx, g.by);
+            rid = (uint32_t)(vbase + dval[slot]);
+        }
+        unsigned long long bal[SB * SB];
+#pragma unroll
+        for (uint32_t b = 0; b < SB * SB; ++b) {
+            bal[b] = __ballot((m >> b) & 1u);
+            if (lane == 0) s_wc[w][b] = (uint32_t)__popcll(bal[b]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t b = 0; b < SB * SB; ++b) {
+            if ((m >> b) & 1u) {
+                uint32_t pos = s_next[b] +
+                               __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[b] >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal[b], 0u));
+                for (uint32_t ww = 0; ww < w; ++ww) pos += s_wc[ww][b];
+                if (pos < pair_cap) {
+                    pval[pos] = rid;
+                    if (pkey) pkey[pos] = g.tile_base + (g.by + b / SB) * g.tiles_x + g.bx + b % SB;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < SB * SB)
+            s_next[threadIdx.x] +=
+                s_wc[0][threadIdx.x] + s_wc[1][threadIdx.x] + s_wc[2][threadIdx.x] + s_wc[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// (d) Tile ranges: tile_start[t] = the first sorted position whose tile id is
+// >= t (tile_start[num_tiles] = the pair count), so tile t's list is
+// [tile_start[t], tile_start[t + 1]).
+// Long sorted lists (more than RANGE_SEARCH_MIN pairs) are indexed by one
+// binary search per tile, whose cost follows the tile count. Shorter ones by
+// one thread per pair that finds a change of tile id and writes the run of
+// tiles that starts there; runs longer than RANGE_SHORT (empty tiles, common
+// in the continuation lists) go to an LDS queue that the whole workgroup
+// writes, so a single thread never walks thousands of tiles.
+constexpr uint64_t RANGE_SEARCH_MIN = 8ull << 20;
+constexpr uint32_t RANGE_SHORT = 16;
+
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ skey,
                                                 const uint32_t *__restrict__ count, uint64_t pair_cap,
                                                 uint32_t num_tiles, uint32_t *__restrict__ tile_start) {
+    __shared__ uint32_t s_lo[260], s_hi[260], s_val[260];  // queued long runs: at most one per thread, plus the end
+    __shared__ uint32_t s_n;
     const uint64_t np = min((uint64_t)*count, pair_cap);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t last_tile = num_tiles - 1;  // keys are tile ids < num_tiles
     if (np == 0) {
         for (uint64_t t = g; t <= num_tiles; t += stride) tile_start[t] = 0;
         return;
     }
-    for (uint64_t p = g; p < np; p += stride) {
-        const uint32_t k = min(skey[p], num_tiles - 1);  // keys are tile ids < num_tiles
-        if (p == 0) {
-            for (uint32_t t = 0; t <= k; ++t) tile_start[t] = 0;
-        } else {
-            const uint32_t prev = min(skey[p - 1], num_tiles - 1);
-            for (uint32_t t = prev + 1; t <= k; ++t) tile_start[t] = (uint32_t)p;
+    if (np > RANGE_SEARCH_MIN) {
+        for (uint64_t t = g; t <= num_tiles; t += stride) {
+            uint64_t lo = 0, hi = np;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (min(skey[mid], last_tile) < t)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            tile_start[t] = (uint32_t)lo;
         }
-        if (p == np - 1)
-            for (uint32_t t = k + 1; t <= num_tiles; ++t) tile_start[t] = (uint32_t)np;
+        return;
+    }
+    auto put_run = [&](uint32_t lo, uint32_t hi, uint32_t val) {  // tiles [lo, hi) start at val
+        if (hi <= lo) return;
+        if (hi - lo <= RANGE_SHORT) {
+            for (uint32_t t = lo; t < hi; ++t) tile_start[t] = val;
+        } else {
+            const uint32_t j = atomicAdd(&s_n, 1u);
+            s_lo[j] = lo;
+            s_hi[j] = hi;
+            s_val[j] = val;
+        }
+    };
+    // Block-uniform loop: every thread reaches the barriers.
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < np; base += stride) {
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        const uint64_t p = base + threadIdx.x;
+        if (p < np) {
+            const uint32_t k = min(skey[p], last_tile);
+            const uint32_t lo = p == 0 ? 0u : min(skey[p - 1], last_tile) + 1u;
+            put_run(lo, k + 1u, (uint32_t)p);
+            if (p == np - 1) put_run(k + 1u, num_tiles + 1u, (uint32_t)np);
+        }
+        __syncthreads();
+        const uint32_t nr = s_n;
+        for (uint32_t i = 0; i < nr; ++i) {
+            const uint32_t hi = s_hi[i], val = s_val[i];
+            for (uint32_t t = s_lo[i] + threadIdx.x; t < hi; t += blockDim.x) tile_start[t] = val;
+        }
+        __syncthreads();
     }
 }
 
@@ -380,8 +658,8 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
                       EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
-                      uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t count_slot,
-                      uint32_t *counters) {
+                      uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval, uint32_t *powner,
+                      uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     if (PHASE == EMIT_FIRST) {
         hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
@@ -389,23 +667,32 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
         const hipError_t eg = hipGetLastError();
         if (eg != hipSuccess) return eg;
     }
+    const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, box, views, split, ip, bcnt, bcnt_all);
+                       vis_count, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr);
     hipError_t e = hipGetLastError();
     // Per view: block offsets, and the view's pair total; then the view bases and the total.
     if (e == hipSuccess) e = launch_scan_rows(s, bcnt, bpv, nviews, nullptr, vtot);
     if (e == hipSuccess) e = launch_scan_rows(s, vtot, nviews, 1, nullptr, counters + count_slot);
     // Unsplit pair count of every view (RenderStats.overlaps).
     if (e == hipSuccess && PHASE != EMIT_SECOND) e = launch_scan_rows(s, bcnt_all, bpv, nviews, nullptr, view_pairs);
+    // Superblock pairs: block offsets, view bases and total.
+    if (e == hipSuccess && bin) e = launch_scan_rows(s, bsb, bpv, nviews, nullptr, vsb);
+    if (e == hipSuccess && bin) e = launch_scan_rows(s, vsb, nviews, 1, nullptr, counters + CNT_SBPAIRS);
     if (e != hipSuccess) return e;
-    if (PHASE == EMIT_SECOND) {
+    if (bin) {
+        const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
+        hipLaunchKernelGGL(k_emit_expand_dense<true>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+                           nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, nullptr, pair_cap, counters);
+    } else if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            vis_count, box, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
     } else {
         const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
-        hipLaunchKernelGGL(k_emit_expand_dense, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv, nviews,
-                           vis_count, box, views, split, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
+        hipLaunchKernelGGL(k_emit_expand_dense<false>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+                           nviews, vis_count, box, views, split, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
     }
     return hipGetLastError();
 }
@@ -417,15 +704,41 @@ uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THRE
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
-                       uint32_t *view_pairs, uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap,
-                       uint32_t *counters) {
+                       uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
+                       uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
     if (phase == EMIT_SECOND)
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt,
-                                       bcnt_all, vtot, view_pairs, pkey, pval, powner, pair_cap, CNT_PAIRS2,
-                                       counters);
+                                       bcnt_all, vtot, view_pairs, nullptr, nullptr, pkey, pval, powner, pair_cap,
+                                       CNT_PAIRS2, counters);
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt, bcnt_all,
-                                  vtot, view_pairs, pkey, pval, powner, pair_cap, CNT_PAIRS, counters);
+                                  vtot, view_pairs, bsb, vsb, pkey, pval, powner, pair_cap, CNT_PAIRS, counters);
 }
+
+hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
+                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *sslot, const uint2 *box,
+                            const uint32_t *dval, uint32_t n_splats, uint32_t *segc, uint32_t *tcount,
+                            uint32_t *trow, uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
+                            const uint32_t *counters) {
+    if (total_tiles == 0 || total_sbs == 0) return launch_fill(s, tile_start, 1, 0u);
+    const uint32_t rows = bin_scan_rows(total_tiles);
+    const dim3 grid(total_sbs, BIN_SEGS);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, views, nviews, sb_start, sslot, box, segc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t pthreads = std::max<uint64_t>((uint64_t)total_sbs * SB * SB, (uint64_t)rows * BIN_SCAN_ROW - total_tiles);
+    const uint32_t pgrid = (uint32_t)std::min<uint64_t>((pthreads + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_bin_prefix, dim3(pgrid), dim3(256), 0, s, views, nviews, total_sbs, segc, tcount,
+                       total_tiles, rows * BIN_SCAN_ROW);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_scan_rows(s, tcount, BIN_SCAN_ROW, rows, nullptr, trow);
+    if (e == hipSuccess) e = launch_scan_rows(s, trow, rows, 1, nullptr, nullptr);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, views, nviews, n_splats, sb_start, sslot, box, dval,
+                       segc, tcount, trow, total_tiles, tile_start, pkey, pval, pair_cap, counters);
+    return hipGetLastError();
+}
+
+uint32_t bin_scan_rows(uint32_t total_tiles) { return (total_tiles + BIN_SCAN_ROW - 1) / BIN_SCAN_ROW; }
 
 hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
                              uint32_t *ip) {

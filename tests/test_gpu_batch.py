@@ -50,6 +50,28 @@ def test_graph_replays_match_eager(oracle):
         assert np.array_equal(imgs[k], cpu.render(v, w, h, (0, 0, 0))[0]), f"new pose view {k}"
 
 
+def test_single_view_graph_matches_eager(oracle):
+    """Without stage timing a single view replays a captured graph: over changing
+    poses, a change of resolution (recapture), capacity growth and a return to the
+    first size, its bytes equal the eager (timed) path's and the cpu backend's."""
+    s = S.random_scene(40000, seed=23, sh_degree=3)
+    views = S.orbit_views(s, 6, 20.0)
+    fast = S.HipRenderer(s, timing=False, initial_pairs=256)
+    timed = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    for k, (w, h) in enumerate([(320, 180), (320, 180), (641, 359), (320, 180), (160, 90), (641, 359)]):
+        case(f"graph single view {k} {w}x{h}")
+        a, sa = fast.render(views[k], w, h, (0.1, 0.0, 0.2))
+        b, sb = timed.render(views[k], w, h, (0.1, 0.0, 0.2))
+        ref, rst = cpu.render(views[k], w, h, (0.1, 0.0, 0.2))
+        assert np.array_equal(a.pixels, b.pixels), f"view {k}: graph replay differs from eager"
+        assert np.array_equal(a.pixels, ref), f"view {k}: graph replay differs from cpu"
+        assert (sa.visible, sa.overlaps) == (rst.visible, rst.overlaps)
+        assert sa.project_ms == 0.0 and sb.total_ms > 0.0
+    fast.close()
+    timed.close()
+
+
 def test_batch_edge_cases(oracle):
     s = S.test_scene("galaxy")
     cam = S.OrbitCamera.for_scene(s)
