@@ -137,6 +137,7 @@ struct sp_handle {
     uint32_t view_cap = 0;
     uint4 *d_rec = nullptr;
     uint2 *d_box = nullptr;  // depth-ordered pixel boxes, view_cap x n
+    uint2 *d_sbox = nullptr; // scene-order pixel boxes written by the projection, view_cap x n
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
     uint32_t *d_vstat = nullptr;  // 4 x view_cap: visible, pairs, emission view bases, superblock-pair bases
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
@@ -253,8 +254,10 @@ GraphEntry &graph_slot(sp_handle *h, const GraphKey &key, bool *hit) {
 // Radix count table large enough for the depth sort of view_cap views and the
 // tile sort of pair_cap pairs.
 int ensure_table(sp_handle *h) {
-    const uint64_t need = std::max(sp::radix_table_words(std::max<uint32_t>(h->view_cap, 1), std::max<uint32_t>(h->n, 1)),
-                                   sp::radix_table_words(1, std::max<uint64_t>(h->pair_cap, 1)));
+    const uint32_t vc = std::max<uint32_t>(h->view_cap, 1);
+    const uint64_t need = std::max({sp::radix_table_words(vc, std::max<uint32_t>(h->n, 1)),
+                                    sp::radix_table_words(1, std::max<uint64_t>(h->pair_cap, 1)),
+                                    sp::radix_table_bound(vc)});
     if (need <= h->table_words) return SP_OK;
     dfree(h, h->d_table, h->table_words * 4);
     h->table_words = 0;
@@ -298,6 +301,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     const uint64_t blocks = std::max(sp::project_blocks(h->n), sp::emit_blocks(h->n));
     dfree(h, h->d_rec, old * nn * 48);
     dfree(h, h->d_box, old * nn * 8);
+    dfree(h, h->d_sbox, old * nn * 8);
     for (int b = 0; b < 2; ++b) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
@@ -316,6 +320,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     const uint64_t v = views;
     int rc = dalloc(h, (void **)&h->d_rec, v * nn * 48);
     if (!rc) rc = dalloc(h, (void **)&h->d_box, v * nn * 8);
+    if (!rc) rc = dalloc(h, (void **)&h->d_sbox, v * nn * 8);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
@@ -488,14 +493,15 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
     HIP_TRY(mark(0));
     // The uncompacted depth keys live in the depth sort's second buffer until the sort needs it.
-    HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_dkey[1], h->d_bcnt, vis, h->d_dkey[0],
+    HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_sbox, h->d_dkey[1], h->d_bcnt, vis,
+                               h->d_dkey[0],
                                h->d_dval[0], nullptr, nullptr, nullptr));
     int di = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di));
     h->depth_buf = di;
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
     const bool binned = mode != BL_DIFF;
-    HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box, h->d_views,
+    HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
                             es, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, binned ? h->d_bsb : nullptr,
                             binned ? vsb : nullptr, h->d_pkey[0], h->d_pval[0], powner, h->pair_cap,
                             h->d_counters));
@@ -531,7 +537,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(mark(3));
     if (split) {
         HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip));
-        HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_rec, h->d_box,
+        HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
                                 h->d_views, es, h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
                                 nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
     }
@@ -796,6 +802,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_soa16);
     (void)hipFree(h->d_rec);
     (void)hipFree(h->d_box);
+    (void)hipFree(h->d_sbox);
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(h->d_dkey[b]);
         (void)hipFree(h->d_dval[b]);
@@ -935,7 +942,7 @@ int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     if (e == hipSuccess) e = hipMemsetAsync(d_bb, 0, n * 8, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h->d_views, h->h_views, sizeof(ViewParams), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
-        e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_dkey[1], h->d_bcnt,
+        e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_sbox, h->d_dkey[1], h->d_bcnt,
                                h->d_vstat, h->d_dkey[0], h->d_dval[0], d_dbg, d_bb, d_valid);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(proj, d_dbg, n * 48, hipMemcpyDeviceToHost);

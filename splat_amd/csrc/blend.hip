@@ -7,9 +7,10 @@
 // record and bounds its falloff over each quadrant), then every wave walks
 // the batch: a record whose pixel box misses the wave's quadrant, or whose
 // falloff cannot reach the splat's alpha threshold at any in-box pixel of the
-// quadrant, is skipped by the whole wave with one LDS read; so is one whose
-// falloff stays below the threshold at every live in-box pixel (those pixels
-// are masked by the reference too).
+// quadrant, is never visited (the wave ballots the reach bits of 64 staged
+// records at a time and walks only the set bits, in list order); one whose
+// falloff stays below the threshold at every live in-box pixel is skipped
+// after the falloff test (those pixels are masked by the reference too).
 // A pixel stops at T < T_MIN; a wave stops when its quadrant has no live
 // pixel, the tile when no pixel is live.
 //
@@ -99,10 +100,18 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         }
         __syncthreads();
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
-        if (__any(live)) {
-            for (uint32_t k = 0; k < nb; ++k) {
-                if (!__any(live)) break;  // every pixel of the quadrant saturated
-                if (!((s_qm[k] >> w) & 1u)) continue;  // wave-uniform: no pixel of the quadrant can blend it
+        // Records of the batch that can reach this wave's quadrant, 64 at a
+        // time: one LDS read per lane and a ballot, then the set bits in order.
+        bool open = __any(live);
+        for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
+            unsigned long long reach = __ballot(g0 + lane_i < nb && ((s_qm[g0 + lane_i] >> w) & 1u));
+            while (reach) {
+                if (!__any(live)) {  // every pixel of the quadrant saturated
+                    open = false;
+                    break;
+                }
+                const uint32_t k = g0 + (uint32_t)__builtin_ctzll(reach);
+                reach &= reach - 1ull;
                 const uint4 r2 = s_rec[k][2];
                 const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
                 const bool act = live && px >= bx0 && px < bx1 && py >= by0 && py < by1;

@@ -20,7 +20,8 @@
 //      with G = dL/dpixel. The nine values are summed over the tile's 256
 //      pixels in a fixed order (DPP sums within a wave, then the four waves) and
 //      stored per pair at its emission index. Records whose falloff cannot
-//      reach a quadrant are skipped by its wave.
+//      reach a quadrant are never visited by its wave (one ballot over the
+//      staged records' reach bits).
 //   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
 //      sums each (view, splat) record's pairs. A per-pair stamp tells the
 //      pairs this backward wrote from stale ones, so nothing is cleared.
@@ -105,9 +106,14 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         }
         for (uint32_t z = threadIdx.x; z < 4u * NG2D * BWD_BATCH; z += BLEND_THREADS) part[z] = 0.0f;
         __syncthreads();
-        for (int k = (int)nb - 1; k >= 0; --k) {
+        // Records that can reach this wave's quadrant (one ballot over the
+        // BWD_BATCH = 64 staged records), walked back to front.
+        static_assert(BWD_BATCH == 64, "one reach ballot per batch");
+        unsigned long long reach = __ballot(lane_i < nb && ((s_qm[lane_i] >> w) & 1u));
+        while (reach) {
+            const int k = 63 - __builtin_clzll(reach);
+            reach &= ~(1ull << k);
             const uint32_t pos = lo + (uint32_t)k;
-            if (!((s_qm[k] >> w) & 1u)) continue;  // wave-uniform: no pixel of the quadrant blended it
             const uint4 r2 = s_rec[k][2];
             const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
             const bool act = pos < last && px >= bx0 && px < bx1 && py >= by0 && py < by1;

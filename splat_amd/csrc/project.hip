@@ -20,11 +20,14 @@ namespace sp {
 namespace {
 
 // Projects every splat into every view. Writes the records of visible
-// splats, full[v*n + i] = depth bits (or FULL_CULLED) and, per view, the
-// block's visible count bcnt[v * gridDim.x + block].
+// splats with their pixel boxes also in sbox[v*n + i] (8 bytes, so that the
+// depth-ordered box gather reads a compact array), full[v*n + i] = depth bits
+// (or FULL_CULLED) and, per view, the block's visible count
+// bcnt[v * gridDim.x + block].
 template <int DEG, int STORE>
 __global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, const ViewParams *__restrict__ views,
                                                              uint32_t nviews, uint4 *__restrict__ rec,
+                                                             uint2 *__restrict__ sbox,
                                                              uint32_t *__restrict__ full,
                                                              uint32_t *__restrict__ bcnt,
                                                              float *__restrict__ dbg,
@@ -48,7 +51,7 @@ __global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, const 
                 uint16_t *gb = (dbg && v == 0) ? dbg_bbox + 4 * (uint64_t)i : nullptr;
                 float depth = 0.0f;
                 vis = project_view<DEG, STORE != SP_STORE_F32>(s, views[v].ctx, rec + 3 * ((uint64_t)v * N + i),
-                                                                &depth, g, gb);
+                                                                &depth, g, gb, sbox + (uint64_t)v * N + i);
                 full[(uint64_t)v * N + i] = vis ? __float_as_uint(depth) : FULL_CULLED;
                 if (dbg_valid && v == 0) dbg_valid[i] = vis ? 1 : 0;
             }
@@ -99,19 +102,21 @@ __global__ __launch_bounds__(PROJECT_THREADS) void k_compact(const uint32_t *__r
 
 template <int DEG, int STORE>
 void launch_deg(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewParams *views, uint32_t nviews,
-                uint4 *rec, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+                uint4 *rec, uint2 *sbox, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox,
+                uint8_t *dbg_valid) {
     hipLaunchKernelGGL((k_project<DEG, STORE>), dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, views, nviews, rec,
-                       full, bcnt, dbg, dbg_bbox, dbg_valid);
+                       sbox, full, bcnt, dbg, dbg_bbox, dbg_valid);
 }
 
 template <int STORE>
 void launch_store(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewParams *views, uint32_t nviews,
-                  uint4 *rec, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+                  uint4 *rec, uint2 *sbox, uint32_t *full, uint32_t *bcnt, float *dbg, uint16_t *dbg_bbox,
+                  uint8_t *dbg_valid) {
     switch (sc.deg) {
-    case 0: launch_deg<0, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
-    case 1: launch_deg<1, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
-    case 2: launch_deg<2, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
-    default: launch_deg<3, STORE>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    case 0: launch_deg<0, STORE>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    case 1: launch_deg<1, STORE>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    case 2: launch_deg<2, STORE>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
+    default: launch_deg<3, STORE>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid); break;
     }
 }
 
@@ -120,21 +125,21 @@ void launch_store(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewPa
 uint32_t project_blocks(uint32_t n) { return n ? (n + PROJECT_THREADS - 1) / PROJECT_THREADS : 1; }
 
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
-                          uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey, uint32_t *dval,
-                          float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+                          uint2 *sbox, uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey,
+                          uint32_t *dval, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
     const uint32_t grid = project_blocks(sc.n);
     switch (sc.storage) {
     case SP_STORE_F16:
-        launch_store<SP_STORE_F16>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
+        launch_store<SP_STORE_F16>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     case SP_STORE_BF16:
-        launch_store<SP_STORE_BF16>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
+        launch_store<SP_STORE_BF16>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     case STORE_PARAMS:
-        launch_store<STORE_PARAMS>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
+        launch_store<STORE_PARAMS>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     default:
-        launch_store<SP_STORE_F32>(s, grid, sc, views, nviews, rec, full, bcnt, dbg, dbg_bbox, dbg_valid);
+        launch_store<SP_STORE_F32>(s, grid, sc, views, nviews, rec, sbox, full, bcnt, dbg, dbg_bbox, dbg_valid);
         break;
     }
     hipError_t e = hipGetLastError();

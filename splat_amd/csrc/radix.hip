@@ -225,8 +225,13 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
 }  // namespace
 
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap) {
-    const uint32_t tk = radix_tile_keys(n_cap);
+    const uint32_t tk = radix_tile_keys(nseg, n_cap);
     return (uint64_t)nseg * RADIX_BINS * ((n_cap + tk - 1) / tk);
+}
+
+uint64_t radix_table_bound(uint32_t max_seg) {
+    const uint64_t small_tile = (uint64_t)RADIX_THREADS * RADIX_ITEMS_SMALL;
+    return (uint64_t)RADIX_BINS * (RADIX_SMALL_MAX / small_tile + max_seg);
 }
 
 namespace {
@@ -245,7 +250,7 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], con
         hipLaunchKernelGGL(k_upsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], seg_counts,
                            seg_stride, n_cap, bps, gps, 8 * p, table);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = launch_scan_digit_rows(s, table, bps, nseg, seg_counts, n_cap, row_total);
+        if (e == hipSuccess) e = launch_scan_digit_rows(s, table, bps, nseg, seg_counts, n_cap, TILE, row_total);
         if (e != hipSuccess) return e;
         const int width = std::min(8, bits - 8 * p);  // digit bits this pass carries
         hipLaunchKernelGGL(k_downsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], vals[cur],
@@ -265,7 +270,7 @@ hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2]
                              uint32_t *table, int *out_index) {
     *out_index = 0;
     if (bits <= 0 || n_cap == 0 || nseg == 0) return hipSuccess;
-    if (radix_tile_keys(n_cap) == RADIX_THREADS * RADIX_ITEMS_SMALL)
+    if (radix_tile_keys(nseg, n_cap) == RADIX_THREADS * RADIX_ITEMS_SMALL)
         return radix_passes<RADIX_ITEMS_SMALL>(s, keys, vals, seg_counts, nseg, seg_stride, n_cap, bits, row_total,
                                                table, out_index);
     return radix_passes<RADIX_ITEMS_LARGE>(s, keys, vals, seg_counts, nseg, seg_stride, n_cap, bits, row_total,

@@ -55,16 +55,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict
 
 // Radix digit rows: row r = (segment g, digit d) of `row_len` per-column
 // counts, of which the first radix_columns(min(seg_counts[g], n_cap),
-// radix_tile_keys(n_cap)) are live.
+// tile_keys) are live.
 // Each live prefix becomes its in-row exclusive scan and row_total[r] its sum
 // (the sort's downsweep turns the 256 row totals of a segment into digit
 // bases itself, so no separate histogram pass is needed).
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
                                                                   const uint32_t *__restrict__ seg_counts,
-                                                                  uint64_t n_cap, uint32_t *__restrict__ row_total) {
+                                                                  uint64_t n_cap, uint32_t tile_keys,
+                                                                  uint32_t *__restrict__ row_total) {
     const uint32_t g = blockIdx.x / RADIX_BINS;
     const uint64_t n = min((uint64_t)seg_counts[g], n_cap);
-    const uint64_t live = min(row_len, (uint64_t)radix_columns(n, radix_tile_keys(n_cap)));
+    const uint64_t live = min(row_len, (uint64_t)radix_columns(n, tile_keys));
     uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
     uint32_t carry = 0;
     for (uint64_t off = 0; off < live; off += SCAN_CHUNK) carry += scan_chunk(row, off, live, carry, true);
@@ -91,10 +92,11 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 }
 
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t *row_total) {
+                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
+                                  uint32_t *row_total) {
     if (nseg == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len,
-                       seg_counts, n_cap, row_total);
+                       seg_counts, n_cap, tile_keys, row_total);
     return hipGetLastError();
 }
 

@@ -25,13 +25,13 @@ constexpr int EMIT_THREADS = 256;     // one depth-ordered splat per thread
 constexpr int BLEND_THREADS = 256;    // one pixel of a 16x16 tile per thread
 constexpr int RADIX_THREADS = 256;    // one digit per thread in the lookback
 // Keys per thread of a radix tile: 16 (4096-key tiles) for large sorts, 4
-// (1024-key tiles) for sorts of up to RADIX_SMALL_MAX keys, whose 4096-key
-// tiles would leave most CUs idle.
+// (1024-key tiles) for sorts whose segments hold at most RADIX_SMALL_MAX keys
+// together, whose 4096-key tiles would leave most CUs idle.
 constexpr int RADIX_ITEMS_LARGE = 16;
 constexpr int RADIX_ITEMS_SMALL = 4;
 constexpr uint64_t RADIX_SMALL_MAX = 16ull << 20;
-__host__ __device__ constexpr uint32_t radix_tile_keys(uint64_t n_cap) {
-    return (uint32_t)(RADIX_THREADS * (n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
+__host__ __device__ constexpr uint32_t radix_tile_keys(uint32_t nseg, uint64_t n_cap) {
+    return (uint32_t)(RADIX_THREADS * ((uint64_t)nseg * n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
 }
 constexpr int RADIX_BINS = 256;       // 8-bit digits
 constexpr uint32_t MAX_VIEWS = 4096;  // views per batch

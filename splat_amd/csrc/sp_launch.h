@@ -15,28 +15,32 @@ namespace sp {
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total);
 // Radix tables: rows (segment g, digit d); only the live prefix of each row
-// (radix_columns(min(seg_counts[g], n_cap), radix_tile_keys(n_cap)) entries) is scanned, in place
-// and from 0; row_total[g * 256 + d] receives the row's sum.
+// (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries) is scanned,
+// in place and from 0; row_total[g * 256 + d] receives the row's sum.
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
-                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t *row_total);
+                                  const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
+                                  uint32_t *row_total);
 // p[0..n) = value (a kernel, so it may be captured in a graph).
 hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 
 // ---- project.hip: (a) projection + SH colour + per-view compaction ----
 uint32_t project_blocks(uint32_t n);
 // Projects every splat into each of `nviews` views: records rec[3*(v*n+i)..]
-// for visible splats; full[v*n + i] depth bits or FULL_CULLED; bcnt (nviews
+// for visible splats and their pixel boxes sbox[v*n + i]; full[v*n + i] depth bits or FULL_CULLED; bcnt (nviews
 // rows of project_blocks(n) block counts, scanned in place); vis_count[v];
 // compacted depth keys dkey[v*n + j] with scene indices dval[v*n + j] for
 // j < vis_count[v], in scene-index order. dbg / dbg_bbox / dbg_valid
 // (optional, n entries) get view 0's full per-splat projection.
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
-                          uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey, uint32_t *dval,
-                          float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
+                          uint2 *sbox, uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey,
+                          uint32_t *dval, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
-// Words of the per-block digit-count table for nseg segments of up to n_cap keys.
+// Words of the per-block digit-count table for nseg segments of up to n_cap
+// keys; radix_table_bound(max_seg) bounds it for every small sort (segments
+// holding at most RADIX_SMALL_MAX keys together) of up to max_seg segments.
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap);
+uint64_t radix_table_bound(uint32_t max_seg);
 // Sorts u32 keys with u32 values over bits [0, bits), 8-bit digits, over
 // `nseg` independent segments: segment g holds min(seg_counts[g], n_cap) keys
 // at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
@@ -57,12 +61,13 @@ uint32_t emit_blocks(uint32_t n);
 // (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
 // also writes each view's unsplit pair count to view_pairs[v]. With powner
 // set, pval[e] = e and powner[e] = record index instead.
-// box: n entries per view, written by FIRST (the depth-ordered pixel boxes) and read by SECOND.
+// box: n entries per view, gathered by FIRST from the scene-order boxes sbox (the
+// depth-ordered pixel boxes) and read by SECOND.
 // Binned FIRST (bsb / vsb given, shaped like bcnt / vtot): the pairs are
 // (global superblock id, depth slot v * n + j) instead, counted in
 // counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts the tile pairs.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                       const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
                        uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
                        uint32_t *powner, uint64_t pair_cap, uint32_t *counters);

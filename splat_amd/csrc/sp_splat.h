@@ -183,7 +183,7 @@ __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat
 // returns true when the splat survives culling.
 template <int DEG, bool PREC>
 __device__ __forceinline__ bool project_view(const Splat<DEG> &s, const ViewCtx &v, uint4 *o, float *depth,
-                                             float *dbg, uint16_t *dbg_bbox) {
+                                             float *dbg, uint16_t *dbg_bbox, uint2 *box = nullptr) {
     const float d0 = s.p[0] - v.eye[0], d1 = s.p[1] - v.eye[1], d2 = s.p[2] - v.eye[2];
     // t = rot * (p - eye), glam column order.
     const float tx = (v.rot[0][0] * d0 + v.rot[0][1] * d1) + v.rot[0][2] * d2;
@@ -265,6 +265,7 @@ __device__ __forceinline__ bool project_view(const Splat<DEG> &s, const ViewCtx 
     // rounding of exp_neg and of the product, so the blend may skip such pixels.
     const float thr = -0.5f * (s.k * s.k) - 0.02f;
     o[2] = make_uint4(__float_as_uint(col[2]), lo, hi, __float_as_uint(thr));
+    if (box) *box = make_uint2(lo, hi);
     *depth = tz;
     if (dbg) {
         const float g[12] = {x, y, ca, cb, cc2, col[0], col[1], col[2], s.opacity, tz, ext_x, ext_y};

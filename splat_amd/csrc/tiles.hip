@@ -152,22 +152,20 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
     }
 }
 
-// box[v*n + j] = pixel box words of view v's j-th depth-ordered splat: one
-// random 8-byte gather per visible splat, so that the emission passes read
-// the boxes in depth order instead of gathering them from the 48-byte records.
+// box[v*n + j] = pixel box words of view v's j-th depth-ordered splat, from
+// the scene-order boxes the projection wrote (sbox, 8 bytes per splat: a
+// view's array stays cache-resident, where the 48-byte records would not), so
+// that the emission passes read the boxes in depth order.
 __global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                                uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                               const uint4 *__restrict__ rec,
+                                                               const uint2 *__restrict__ sbox,
                                                                uint2 *__restrict__ box) {
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);
     const uint32_t j = local * EMIT_THREADS + threadIdx.x;
     if (j >= nvis) return;
-    const uint2 *r = reinterpret_cast<const uint2 *>(rec + 3 * (vbase + dval[vbase + j]) + 2);
-    const uint2 w01 = r[0];  // words 8-9 of the record: blue, x0|y0
-    const uint2 w23 = r[1];  // words 10-11: x1|y1, threshold
-    box[vbase + j] = make_uint2(w01.y, w23.x);
+    box[vbase + j] = sbox[vbase + dval[vbase + j]];
 }
 
 // Pair-balanced expansion: the phase's pairs (views concatenated) are cut
@@ -655,7 +653,7 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
 
 template <int PHASE>
 hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                      const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                      const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                       EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval, uint32_t *powner,
@@ -663,7 +661,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     const uint32_t bpv = emit_blocks(n_splats);
     if (PHASE == EMIT_FIRST) {
         hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           vis_count, rec, box);
+                           vis_count, sbox, box);
         const hipError_t eg = hipGetLastError();
         if (eg != hipSuccess) return eg;
     }
@@ -702,15 +700,15 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
 uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THREADS : 1; }
 
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
-                       const uint32_t *vis_count, const uint4 *rec, uint2 *box, const ViewParams *views,
+                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
                        uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
                        uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
     if (phase == EMIT_SECOND)
-        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt,
+        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt,
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, pkey, pval, powner, pair_cap,
                                        CNT_PAIRS2, counters);
-    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, rec, box, views, split, ip, bcnt, bcnt_all,
+    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
                                   vtot, view_pairs, bsb, vsb, pkey, pval, powner, pair_cap, CNT_PAIRS, counters);
 }
 
