@@ -139,7 +139,8 @@ struct sp_handle {
     uint2 *d_box = nullptr;  // depth-ordered pixel boxes, view_cap x n
     uint2 *d_sbox = nullptr; // scene-order pixel boxes written by the projection, view_cap x n
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
-    uint32_t *d_vstat = nullptr;  // 4 x view_cap: visible, pairs, emission view bases, superblock-pair bases
+    uint32_t *d_vstat = nullptr;  // 5 x view_cap: visible, pairs, emission view bases, superblock-pair
+                                  // bases, unfinished tiles after the first phase
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
     uint32_t *d_bcnt_all = nullptr;  // view_cap x block counts of the unsplit emission
     uint32_t *d_bsb = nullptr;    // view_cap x block counts of superblock pairs (binned first phase)
@@ -306,7 +307,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
     }
-    dfree(h, h->d_vstat, old * 16);
+    dfree(h, h->d_vstat, old * 20);
     dfree(h, h->d_bcnt, old * blocks * 4);
     dfree(h, h->d_bcnt_all, old * blocks * 4);
     dfree(h, h->d_bsb, old * blocks * 4);
@@ -325,7 +326,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
     }
-    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 16);
+    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 20);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt_all, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bsb, v * blocks * 4);
@@ -500,9 +501,11 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di));
     h->depth_buf = di;
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
+    uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
     const bool binned = mode != BL_DIFF;
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
-                            es, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, binned ? h->d_bsb : nullptr,
+                            es, nullptr, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs,
+                            binned ? h->d_bsb : nullptr,
                             binned ? vsb : nullptr, h->d_pkey[0], h->d_pval[0], powner, h->pair_cap,
                             h->d_counters));
     HIP_TRY(mark(1));
@@ -536,9 +539,9 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     }
     HIP_TRY(mark(3));
     if (split) {
-        HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip));
+        HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip, vopen));
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
-                                h->d_views, es, h->d_ip, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
+                                h->d_views, es, h->d_ip, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
                                 nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));

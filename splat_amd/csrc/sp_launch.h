@@ -55,7 +55,8 @@ uint32_t emit_blocks(uint32_t n);
 // One (global tile id, record index) pair per overlapped 16x16 tile of each
 // view's depth-ordered splats, views in order, for emission phase `phase`
 // (EMIT_FIRST: the nearest K1 splats of each view, every tile; EMIT_SECOND:
-// the rest, only tiles with ip's row prefix counting them unfinished).
+// the rest, only tiles with ip's row prefix counting them unfinished; vopen[v]
+// = 0 skips view v).
 // bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
 // entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
 // (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
@@ -68,9 +69,9 @@ uint32_t emit_blocks(uint32_t n);
 // counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts the tile pairs.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
-                       uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
-                       uint32_t *powner, uint64_t pair_cap, uint32_t *counters);
+                       EmitSplit split, const uint32_t *ip, const uint32_t *vopen, uint32_t *bcnt,
+                       uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
+                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters);
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
 // entries, trow bin_scan_rows.
 uint32_t bin_scan_rows(uint32_t total_tiles);
@@ -83,9 +84,10 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
                             const uint32_t *dval, uint32_t n_splats, uint32_t *segc, uint32_t *tcount,
                             uint32_t *trow, uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters);
-// Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every view.
+// Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every
+// view, and each view's unfinished-tile count vopen[v] (read by SECOND).
 hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
-                             uint32_t *ip);
+                             uint32_t *ip, uint32_t *vopen);
 // tile_start[0..num_tiles] from the tile-sorted pair keys; *count pairs.
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *count, uint64_t pair_cap,
                          uint32_t num_tiles, uint32_t *tile_start);

@@ -128,8 +128,13 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
                                                              const uint32_t *__restrict__ ip,
                                                              uint32_t *__restrict__ bcnt,
                                                              uint32_t *__restrict__ bcnt_all,
-                                                             uint32_t *__restrict__ bsb) {
+                                                             uint32_t *__restrict__ bsb,
+                                                             const uint32_t *__restrict__ vopen) {
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
+    if (PHASE == EMIT_SECOND && vopen[v] == 0) {  // block-uniform: the first phase finished the view
+        if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+        return;
+    }
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);  // dval holds n_splats entries per view
     const uint32_t k1 = first_count(nvis, split);
@@ -310,6 +315,15 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
     __shared__ uint32_t s_id[EMIT_THREADS];
     __shared__ uint32_t s_rows[EMIT_THREADS];
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && counters[CNT_PAIRS2] > pair_cap) counters[CNT_OVERFLOW] = 1u;
+    // Blocks without pairs (most: the first phase finished their tiles) leave
+    // before deriving any rectangle; the scanned offsets give the block total.
+    {
+        const uint32_t nv = gridDim.x / bpv;
+        const uint32_t vend = v + 1 < nv ? vbase_pairs[v + 1] : counters[CNT_PAIRS2];
+        const uint32_t bend = local + 1 < bpv ? boff[blockIdx.x + 1] : vend - vbase_pairs[v];
+        if (bend == boff[blockIdx.x]) return;  // block-uniform
+    }
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);
     const uint32_t k1 = first_count(nvis, split);
@@ -324,7 +338,6 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
     s_rw[threadIdx.x] = r.rw;
     s_id[threadIdx.x] = r.id;
     s_rows[threadIdx.x] = r.ty1 - r.ty0;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && counters[CNT_PAIRS2] > pair_cap) counters[CNT_OVERFLOW] = 1u;
     __syncthreads();
     const uint64_t base = (uint64_t)vbase_pairs[v] + boff[blockIdx.x];
     for (uint32_t q = threadIdx.x; q < total; q += EMIT_THREADS) {
@@ -368,12 +381,22 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
 }
 
 // Inclusive per-row prefix counts of unfinished tiles: one wave per tile
-// row (lanes over 64 columns at a time), blockIdx.y = view.
+// row (lanes over 64 columns at a time), blockIdx.y = view. Block 0 of each
+// view also counts the view's unfinished tiles into vopen[v], so that the
+// second phase passes over views the first phase finished.
 __global__ __launch_bounds__(256) void k_row_prefix(const ViewParams *__restrict__ views,
-                                                    const uint32_t *__restrict__ done, uint32_t *__restrict__ ip) {
+                                                    const uint32_t *__restrict__ done, uint32_t *__restrict__ ip,
+                                                    uint32_t *__restrict__ vopen) {
     const ViewParams &vp = views[blockIdx.y];
     const uint32_t tx_n = vp.ctx.tiles_x, ty_n = vp.ctx.tiles_y;
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    if (blockIdx.x == 0) {
+        uint32_t open = 0;
+        for (uint32_t t = threadIdx.x; t < tx_n * ty_n; t += 256) open += done[vp.tile_base + t] ? 0u : 1u;
+        uint32_t total;
+        block_exclusive_scan<256>(open, &total);
+        if (threadIdx.x == 0) vopen[blockIdx.y] = total;
+    }
     for (uint32_t ty = blockIdx.x * 4 + w; ty < ty_n; ty += gridDim.x * 4) {
         const uint32_t rb = vp.tile_base + ty * tx_n;
         uint32_t carry = 0;
@@ -656,8 +679,8 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                       EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
-                      uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval, uint32_t *powner,
-                      uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
+                      uint32_t *bsb, uint32_t *vsb, const uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
+                      uint32_t *powner, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     if (PHASE == EMIT_FIRST) {
         hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
@@ -667,7 +690,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     }
     const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr);
+                       vis_count, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
     hipError_t e = hipGetLastError();
     // Per view: block offsets, and the view's pair total; then the view bases and the total.
     if (e == hipSuccess) e = launch_scan_rows(s, bcnt, bpv, nviews, nullptr, vtot);
@@ -701,15 +724,16 @@ uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THRE
 
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot,
-                       uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb, uint32_t *pkey, uint32_t *pval,
-                       uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
+                       EmitSplit split, const uint32_t *ip, const uint32_t *vopen, uint32_t *bcnt,
+                       uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
+                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
     if (phase == EMIT_SECOND)
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt,
-                                       bcnt_all, vtot, view_pairs, nullptr, nullptr, pkey, pval, powner, pair_cap,
-                                       CNT_PAIRS2, counters);
+                                       bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, powner,
+                                       pair_cap, CNT_PAIRS2, counters);
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
-                                  vtot, view_pairs, bsb, vsb, pkey, pval, powner, pair_cap, CNT_PAIRS, counters);
+                                  vtot, view_pairs, bsb, vsb, nullptr, pkey, pval, powner, pair_cap, CNT_PAIRS,
+                                  counters);
 }
 
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
@@ -739,9 +763,9 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
 uint32_t bin_scan_rows(uint32_t total_tiles) { return (total_tiles + BIN_SCAN_ROW - 1) / BIN_SCAN_ROW; }
 
 hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
-                             uint32_t *ip) {
+                             uint32_t *ip, uint32_t *vopen) {
     if (nviews == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_row_prefix, dim3(16, nviews), dim3(256), 0, s, views, done, ip);
+    hipLaunchKernelGGL(k_row_prefix, dim3(16, nviews), dim3(256), 0, s, views, done, ip, vopen);
     return hipGetLastError();
 }
 
