@@ -515,20 +515,19 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         const int sb_bits = (int)bits_for(total_sbs ? total_sbs - 1 : 0);
         HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_SBPAIRS, 1, 0, h->pair_cap,
                                       sb_bits, h->d_rowtot, h->d_table, &si));
-        HIP_TRY(sp::launch_ranges(s, h->d_pkey[si], h->d_counters + sp::CNT_SBPAIRS, h->pair_cap, total_sbs,
-                                  h->d_sb_start));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[si], (1u << sp::SB_SUB_SHIFT) - 1u, h->d_counters + sp::CNT_SBPAIRS,
+                                  h->pair_cap, total_sbs, h->d_sb_start));
         pi = 1 - si;
         // Tile ids are kept only when the lists are read back (no blend).
-        HIP_TRY(sp::launch_bin_lists(s, h->d_views, nviews, total_sbs, total_tiles, h->d_sb_start, h->d_pval[si],
-                                     h->d_box, h->d_dval[di], h->n, h->d_segc, h->d_tcount, h->d_trow,
-                                     h->d_tile_start,
+        HIP_TRY(sp::launch_bin_lists(s, h->d_views, nviews, total_sbs, total_tiles, h->d_sb_start, h->d_pkey[si],
+                                     h->d_pval[si], h->d_segc, h->d_tcount, h->d_trow, h->d_tile_start,
                                      mode == BL_NONE ? h->d_pkey[pi] : nullptr, h->d_pval[pi], h->pair_cap,
                                      h->d_counters));
     } else {
         HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
                                       tile_bits, h->d_rowtot, h->d_table, &pi));
-        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS, h->pair_cap, total_tiles,
-                                  h->d_tile_start));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], 0xFFFFFFFFu, h->d_counters + sp::CNT_PAIRS, h->pair_cap,
+                                  total_tiles, h->d_tile_start));
     }
     HIP_TRY(mark(2));
     if (mode != BL_NONE) {
@@ -548,8 +547,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     if (split) {
         HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS2, 1, 0, h->pair_cap,
                                       tile_bits, h->d_rowtot, h->d_table, &pi));
-        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], h->d_counters + sp::CNT_PAIRS2, h->pair_cap, total_tiles,
-                                  h->d_tile_start));
+        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], 0xFFFFFFFFu, h->d_counters + sp::CNT_PAIRS2, h->pair_cap,
+                                  total_tiles, h->d_tile_start));
     }
     HIP_TRY(mark(5));
     if (split)
@@ -593,6 +592,8 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     const uint32_t total_tiles = (uint32_t)total;
     rc = ensure_tiles(h, total_tiles);
     if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
+    if (!rc && mode != BL_DIFF && total_sb >= (1ull << sp::SB_SUB_SHIFT))
+        return fail(SP_E_INVALID, "too many superblocks in one batch");
     if (!rc && mode != BL_DIFF) rc = ensure_bin(h, total_tiles, total_sb);
     if (!rc && mode == BL_RGB && h->split_shift > 0) rc = ensure_split(h, total_tiles);
     if (rc) return rc;

@@ -44,8 +44,10 @@ template <int ITEMS>
 __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ seg_counts,
                                                            uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
-                                                           uint32_t gps, int shift, uint32_t *__restrict__ table) {
+                                                           uint32_t gps, int shift, int width,
+                                                           uint32_t *__restrict__ table) {
     __shared__ uint32_t s_h[NWAVES][RADIX_BINS];
+    const uint32_t dmask = (1u << width) - 1u;
     const uint32_t seg = blockIdx.x / gps;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
                 const uint32_t i = w * WAVE_KEYS + k * 64 + lane;
-                dg[k] = i < rem ? (kt[i] >> shift) & 0xFFu : RADIX_BINS;  // RADIX_BINS: no key
+                dg[k] = i < rem ? (kt[i] >> shift) & dmask : RADIX_BINS;  // RADIX_BINS: no key
             }
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
@@ -115,6 +117,7 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     const uint32_t *kt = ks + base;
     const uint32_t *vt = vs + base;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t dmask = (1u << width) - 1u;
     for (uint32_t k = threadIdx.x; k < NWAVES * RADIX_BINS; k += RADIX_THREADS) (&s_wh[0][0])[k] = 0;
     __syncthreads();
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
@@ -128,10 +131,11 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const bool valid = w * WAVE_KEYS + k * 64 + lane < rem;
-        const uint32_t d = (key[k] >> shift) & 0xFFu;
+        const uint32_t d = (key[k] >> shift) & dmask;
         // m = the lanes holding this lane's digit. A wave whose keys share
         // one digit needs no bit tests; otherwise one ballot per bit that the
-        // pass carries (keys are below 2^bits, so higher digit bits are 0).
+        // pass carries (digits are the key's bits [shift, shift + width): bits
+        // at and above `bits` do not order the keys and may carry data).
         unsigned long long m = __ballot(valid);
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         if (!__all(!valid || d == d0)) {
@@ -170,14 +174,14 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         if (w * WAVE_KEYS + k * 64 + lane < rem) {
-            const uint32_t dk = (key[k] >> shift) & 0xFFu;
+            const uint32_t dk = (key[k] >> shift) & dmask;
             s_ex[ex_slot(s_lstart[dk] + s_wh[w][dk] + rank[k])] = make_uint2(key[k], val[k]);
         }
     }
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < rem; j += RADIX_THREADS) {
         const uint2 e = s_ex[ex_slot(j)];
-        const uint32_t pos = s_gbase[(e.x >> shift) & 0xFFu] + j;
+        const uint32_t pos = s_gbase[(e.x >> shift) & dmask] + j;
         if (pos < n) {  // always true for consistent counts; guards the buffer
             ko[pos] = e.x;
             vo[pos] = e.y;
@@ -247,12 +251,12 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], con
     // over the live columns, so a capacity far above the live count costs little.
     const uint32_t gps = std::max<uint32_t>(1, std::min<uint32_t>(bps, std::max<uint32_t>(1, RADIX_GRID / nseg)));
     for (int p = 0; p < passes; ++p) {
+        const int width = std::min(8, bits - 8 * p);  // digit bits this pass carries
         hipLaunchKernelGGL(k_upsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], seg_counts,
-                           seg_stride, n_cap, bps, gps, 8 * p, table);
+                           seg_stride, n_cap, bps, gps, 8 * p, width, table);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = launch_scan_digit_rows(s, table, bps, nseg, seg_counts, n_cap, TILE, row_total);
         if (e != hipSuccess) return e;
-        const int width = std::min(8, bits - 8 * p);  // digit bits this pass carries
         hipLaunchKernelGGL(k_downsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], vals[cur],
                            keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps, 8 * p, width,
                            table, row_total);

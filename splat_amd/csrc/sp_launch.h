@@ -41,7 +41,8 @@ hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *v
 // holding at most RADIX_SMALL_MAX keys together) of up to max_seg segments.
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap);
 uint64_t radix_table_bound(uint32_t max_seg);
-// Sorts u32 keys with u32 values over bits [0, bits), 8-bit digits, over
+// Sorts u32 keys with u32 values by bits [0, bits) (stable; higher key bits
+// travel with the key without ordering it), 8-bit digits, over
 // `nseg` independent segments: segment g holds min(seg_counts[g], n_cap) keys
 // at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
 // between keys[0]/keys[1]; *out_index names the buffer with the result.
@@ -65,8 +66,9 @@ uint32_t emit_blocks(uint32_t n);
 // box: n entries per view, gathered by FIRST from the scene-order boxes sbox (the
 // depth-ordered pixel boxes) and read by SECOND.
 // Binned FIRST (bsb / vsb given, shaped like bcnt / vtot): the pairs are
-// (global superblock id, depth slot v * n + j) instead, counted in
-// counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts the tile pairs.
+// (global superblock id | sub-rectangle << SB_SUB_SHIFT, record index)
+// instead, counted in counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts
+// the tile pairs.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, const uint32_t *vopen, uint32_t *bcnt,
@@ -75,22 +77,24 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
 // entries, trow bin_scan_rows.
 uint32_t bin_scan_rows(uint32_t total_tiles);
-// Binned FIRST lists: from the superblock-sorted depth slots sslot (ranges
+// Binned FIRST lists: from the superblock-sorted pairs (skey = superblock id
+// | sub-rectangle << SB_SUB_SHIFT, sval = record index; ranges
 // sb_start[0..total_sbs]), the tile lists in tile order (pval = record index,
 // pkey = tile id when pkey is given) and tile_start[0..total_tiles]. segc:
 // total_sbs * BIN_SEGS * SB * SB scratch counts.
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
-                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *sslot, const uint2 *box,
-                            const uint32_t *dval, uint32_t n_splats, uint32_t *segc, uint32_t *tcount,
-                            uint32_t *trow, uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
+                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *skey,
+                            const uint32_t *sval, uint32_t *segc, uint32_t *tcount, uint32_t *trow,
+                            uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters);
 // Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every
 // view, and each view's unfinished-tile count vopen[v] (read by SECOND).
 hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
                              uint32_t *ip, uint32_t *vopen);
-// tile_start[0..num_tiles] from the tile-sorted pair keys; *count pairs.
-hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *count, uint64_t pair_cap,
-                         uint32_t num_tiles, uint32_t *tile_start);
+// tile_start[0..num_tiles] from the sorted pair keys (tile or superblock id =
+// key & key_mask); *count pairs.
+hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask, const uint32_t *count,
+                         uint64_t pair_cap, uint32_t num_tiles, uint32_t *tile_start);
 
 // ---- blend.hip: (e) front-to-back blend of every tile of every view ----
 // mode BM_RGB / BM_FIRST / BM_SECOND write RGB8 (ViewParams::out); BM_DIFF

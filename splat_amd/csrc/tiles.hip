@@ -89,6 +89,28 @@ __device__ __forceinline__ SbRect sb_rect(const Rect &r, uint32_t sbx, uint32_t 
     return q;
 }
 
+// Sub-rectangle of the tile rectangle [tx0, tx1) x [ty0, ty1) inside the
+// superblock whose first tile is (bx, by), in 8 bits: first column, last
+// column, first row, last row, 2 bits each (the rectangle meets the
+// superblock).
+__device__ __forceinline__ uint32_t sb_sub(uint32_t tx0, uint32_t ty0, uint32_t tx1, uint32_t ty1, uint32_t bx,
+                                           uint32_t by) {
+    const uint32_t i0 = tx0 > bx ? tx0 - bx : 0u, i1 = min(tx1 - bx, SB) - 1u;
+    const uint32_t j0 = ty0 > by ? ty0 - by : 0u, j1 = min(ty1 - by, SB) - 1u;
+    return i0 | (i1 << 2) | (j0 << 4) | (j1 << 6);
+}
+
+// The 16-bit tile mask (bit row * SB + column) of a sub-rectangle code.
+__device__ __forceinline__ uint32_t sb_sub_mask(uint32_t sub) {
+    const uint32_t i0 = sub & 3u, i1 = (sub >> 2) & 3u, j0 = (sub >> 4) & 3u, j1 = (sub >> 6) & 3u;
+    const uint32_t row = ((2u << i1) - 1u) & ~((1u << i0) - 1u);
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t jj = 0; jj < SB; ++jj)
+        if (jj >= j0 && jj <= j1) m |= row << (SB * jj);
+    return m;
+}
+
 // Number of depth-ordered splats of a view the FIRST phase emits.
 __device__ __forceinline__ uint32_t first_count(uint32_t vis, EmitSplit sp) {
     const uint32_t f = (uint32_t)(((uint64_t)vis + (1ull << sp.shift) - 1) >> sp.shift);
@@ -183,8 +205,9 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *_
 // leave one workgroup with most of the pairs.
 //
 // BIN: the binned first phase expands each splat's superblock rectangle
-// instead; the pair carries the splat's depth slot (v * n + j) rather than its
-// record, and the pair count comes from counters[CNT_SBPAIRS].
+// instead: key = global superblock id | sub-rectangle << SB_SUB_SHIFT (the
+// part of the splat's tile rectangle inside the superblock, sb_sub), value =
+// record index; the pair count comes from counters[CNT_SBPAIRS].
 template <bool BIN>
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                               uint32_t bpv, uint32_t nviews,
@@ -200,6 +223,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
     __shared__ uint32_t s_org[EMIT_THREADS];  // first global tile id of the rectangle
     __shared__ uint32_t s_rw[EMIT_THREADS];   // rectangle width in tiles
     __shared__ uint32_t s_id[EMIT_THREADS];   // record index (view * n + scene index)
+    __shared__ uint32_t s_t0[BIN ? EMIT_THREADS : 1], s_t1[BIN ? EMIT_THREADS : 1];  // tile rectangle (BIN)
     __shared__ uint32_t s_v, s_blk;
     const uint64_t np = counters[BIN ? CNT_SBPAIRS : CNT_PAIRS];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -257,7 +281,11 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
             s_pre[threadIdx.x] = lscan;
             s_org[threadIdx.x] = org;
             s_rw[threadIdx.x] = rw;
-            s_id[threadIdx.x] = BIN ? (uint32_t)(vbase + j) : r.id;
+            s_id[threadIdx.x] = r.id;
+            if (BIN) {
+                s_t0[threadIdx.x] = r.tx0 | (r.ty0 << 16);
+                s_t1[threadIdx.x] = r.tx1 | (r.ty1 << 16);
+            }
             __syncthreads();
             const uint64_t bbase = (uint64_t)vbase_pairs[v] + boff[(uint64_t)v * bpv + blk];
             const uint64_t hi = min(c1, bbase + total);
@@ -276,8 +304,15 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
                 const uint32_t rwl = s_rw[lo];
                 const uint32_t ry = loc / rwl, rx = loc - ry * rwl;
                 const uint32_t tile = s_org[lo] + ry * gw + rx;
+                if (BIN) {
+                    const uint32_t t0 = s_t0[lo], t1 = s_t1[lo];
+                    const uint32_t bx = ((t0 & 0xFFFFu) / SB + rx) * SB, by = ((t0 >> 16) / SB + ry) * SB;
+                    pkey[qq] = tile | (sb_sub(t0 & 0xFFFFu, t0 >> 16, t1 & 0xFFFFu, t1 >> 16, bx, by) << SB_SUB_SHIFT);
+                    pval[qq] = s_id[lo];
+                    continue;
+                }
                 pkey[qq] = tile;
-                if (!BIN && powner) {
+                if (powner) {
                     // Differentiable path: the pair carries its emission index; the
                     // record index stays behind in emission order.
                     pval[qq] = (uint32_t)qq;
@@ -435,20 +470,6 @@ __device__ __forceinline__ uint32_t view_of_sb(const ViewParams *views, uint32_t
     return lo;
 }
 
-// Tiles of superblock origin (bx, by) covered by a splat's pixel box, as bits
-// (row * SB + column); the tile rectangle is splat_rect's.
-__device__ __forceinline__ uint32_t sb_mask(uint2 b, uint32_t bx, uint32_t by) {
-    const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
-    if (!(x0 < x1 && y0 < y1)) return 0u;
-    const int i0 = max((int)(x0 / TILE) - (int)bx, 0), i1 = min((int)((x1 + TILE - 1) / TILE) - (int)bx, (int)SB);
-    const int j0 = max((int)(y0 / TILE) - (int)by, 0), j1 = min((int)((y1 + TILE - 1) / TILE) - (int)by, (int)SB);
-    if (i0 >= i1 || j0 >= j1) return 0u;
-    const uint32_t row = ((1u << i1) - 1u) & ~((1u << i0) - 1u);
-    uint32_t m = 0;
-    for (int jj = j0; jj < j1; ++jj) m |= row << (SB * jj);
-    return m;
-}
-
 struct SbGeom {
     uint32_t v, bx, by, tiles_x, tiles_y, tile_base;
 };
@@ -476,21 +497,18 @@ __device__ __forceinline__ void bin_segment(uint32_t e0, uint32_t e1, uint32_t y
 }
 
 // segc[(sb * BIN_SEGS + y) * 16 + b] = entries of segment y of superblock sb
-// that cover its tile b.
-__global__ __launch_bounds__(256) void k_bin_count(const ViewParams *__restrict__ views, uint32_t nviews,
-                                                   const uint32_t *__restrict__ sb_start,
-                                                   const uint32_t *__restrict__ sslot, const uint2 *__restrict__ box,
-                                                   uint32_t *__restrict__ segc) {
+// that cover its tile b (from the sub-rectangles in the sorted keys).
+__global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ sb_start,
+                                                   const uint32_t *__restrict__ skey, uint32_t *__restrict__ segc) {
     __shared__ uint32_t s_c[4][SB * SB];
     const uint32_t sb = blockIdx.x, y = blockIdx.y, w = threadIdx.x >> 6, lane = lane_id();
     uint32_t s0, s1;
     bin_segment(sb_start[sb], sb_start[sb + 1], y, s0, s1);
     uint32_t mine = 0;  // lane b < 16: this wave's entries covering tile b
     if (s0 < s1) {
-        const SbGeom g = sb_geom(views, nviews, sb);
         for (uint32_t c = s0; c < s1; c += 256) {  // block-uniform
             const uint32_t e = c + threadIdx.x;
-            const uint32_t m = e < s1 ? sb_mask(box[sslot[e]], g.bx, g.by) : 0u;
+            const uint32_t m = e < s1 ? sb_sub_mask(skey[e] >> SB_SUB_SHIFT) : 0u;
 #pragma unroll
             for (uint32_t b = 0; b < SB * SB; ++b) {
                 const uint32_t k = (uint32_t)__popcll(__ballot((m >> b) & 1u));
@@ -537,9 +555,9 @@ __global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict
 // BIN_SCAN_ROW tiles); trow: exclusive prefix of the row totals; segc: the
 // segment offsets within each tile's list.
 __global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict__ views, uint32_t nviews,
-                                                   uint32_t n_splats, const uint32_t *__restrict__ sb_start,
-                                                   const uint32_t *__restrict__ sslot, const uint2 *__restrict__ box,
-                                                   const uint32_t *__restrict__ dval,
+                                                   const uint32_t *__restrict__ sb_start,
+                                                   const uint32_t *__restrict__ skey,
+                                                   const uint32_t *__restrict__ sval,
                                                    const uint32_t *__restrict__ segc,
                                                    const uint32_t *__restrict__ tscan,
                                                    const uint32_t *__restrict__ trow, uint32_t total_tiles,
@@ -555,7 +573,6 @@ __global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict_
         tile_start[total_tiles] = (uint32_t)min((uint64_t)counters[CNT_PAIRS], pair_cap);
     if (s0 >= s1 && y > 0) return;  // block-uniform; segment 0 always writes the tile starts
     const SbGeom g = sb_geom(views, nviews, sb);
-    const uint64_t vbase = (uint64_t)g.v * n_splats;
     if (threadIdx.x < SB * SB) {
         const uint32_t tx = g.bx + (threadIdx.x % SB), ty = g.by + (threadIdx.x / SB);
         uint32_t start = 0;
@@ -571,9 +588,8 @@ __global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict_
         const uint32_t e = c + threadIdx.x;
         uint32_t m = 0, rid = 0;
         if (e < s1) {
-            const uint32_t slot = sslot[e];
-            m = sb_mask(box[slot], g.bx, g.by);
-            rid = (uint32_t)(vbase + dval[slot]);
+            m = sb_sub_mask(skey[e] >> SB_SUB_SHIFT);
+            rid = sval[e];
         }
         unsigned long long bal[SB * SB];
 #pragma unroll
@@ -615,7 +631,7 @@ __global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict_
 constexpr uint64_t RANGE_SEARCH_MIN = 8ull << 20;
 constexpr uint32_t RANGE_SHORT = 16;
 
-__global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ skey,
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ skey, uint32_t key_mask,
                                                 const uint32_t *__restrict__ count, uint64_t pair_cap,
                                                 uint32_t num_tiles, uint32_t *__restrict__ tile_start) {
     __shared__ uint32_t s_lo[260], s_hi[260], s_val[260];  // queued long runs: at most one per thread, plus the end
@@ -633,7 +649,7 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
             uint64_t lo = 0, hi = np;
             while (lo < hi) {
                 const uint64_t mid = (lo + hi) >> 1;
-                if (min(skey[mid], last_tile) < t)
+                if (min(skey[mid] & key_mask, last_tile) < t)
                     lo = mid + 1;
                 else
                     hi = mid;
@@ -659,8 +675,8 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
         __syncthreads();
         const uint64_t p = base + threadIdx.x;
         if (p < np) {
-            const uint32_t k = min(skey[p], last_tile);
-            const uint32_t lo = p == 0 ? 0u : min(skey[p - 1], last_tile) + 1u;
+            const uint32_t k = min(skey[p] & key_mask, last_tile);
+            const uint32_t lo = p == 0 ? 0u : min(skey[p - 1] & key_mask, last_tile) + 1u;
             put_run(lo, k + 1u, (uint32_t)p);
             if (p == np - 1) put_run(k + 1u, num_tiles + 1u, (uint32_t)np);
         }
@@ -737,14 +753,14 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
 }
 
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
-                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *sslot, const uint2 *box,
-                            const uint32_t *dval, uint32_t n_splats, uint32_t *segc, uint32_t *tcount,
-                            uint32_t *trow, uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
+                            uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *skey,
+                            const uint32_t *sval, uint32_t *segc, uint32_t *tcount, uint32_t *trow,
+                            uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters) {
     if (total_tiles == 0 || total_sbs == 0) return launch_fill(s, tile_start, 1, 0u);
     const uint32_t rows = bin_scan_rows(total_tiles);
     const dim3 grid(total_sbs, BIN_SEGS);
-    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, views, nviews, sb_start, sslot, box, segc);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, sb_start, skey, segc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint64_t pthreads = std::max<uint64_t>((uint64_t)total_sbs * SB * SB, (uint64_t)rows * BIN_SCAN_ROW - total_tiles);
@@ -755,8 +771,8 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
     if (e == hipSuccess) e = launch_scan_rows(s, tcount, BIN_SCAN_ROW, rows, nullptr, trow);
     if (e == hipSuccess) e = launch_scan_rows(s, trow, rows, 1, nullptr, nullptr);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, views, nviews, n_splats, sb_start, sslot, box, dval,
-                       segc, tcount, trow, total_tiles, tile_start, pkey, pval, pair_cap, counters);
+    hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, views, nviews, sb_start, skey, sval, segc, tcount,
+                       trow, total_tiles, tile_start, pkey, pval, pair_cap, counters);
     return hipGetLastError();
 }
 
@@ -769,11 +785,12 @@ hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nv
     return hipGetLastError();
 }
 
-hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, const uint32_t *count, uint64_t pair_cap,
-                         uint32_t num_tiles, uint32_t *tile_start) {
+hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask, const uint32_t *count,
+                         uint64_t pair_cap, uint32_t num_tiles, uint32_t *tile_start) {
     const uint64_t work = pair_cap > num_tiles ? pair_cap : (uint64_t)num_tiles + 1;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((work + 255) / 256, 4096);  // grid-stride over the pairs
-    hipLaunchKernelGGL(k_ranges, dim3(grid), dim3(256), 0, s, skey, count, pair_cap, num_tiles, tile_start);
+    hipLaunchKernelGGL(k_ranges, dim3(grid), dim3(256), 0, s, skey, key_mask, count, pair_cap, num_tiles,
+                       tile_start);
     return hipGetLastError();
 }
 
