@@ -202,7 +202,7 @@ def main() -> None:
 
     if not args.no_batch:
         try:
-            result["batched"] = bench_batched(S, torch, scene, args.batch_views)
+            result["batched"] = bench_batched(S, torch, scene, args.batch_views, not args.no_autotune)
         except Exception as e:  # reported, not fatal for the headline number
             result["batched"] = {"error": repr(e)}
 
@@ -397,7 +397,7 @@ def bench_precision(S, torch, args, W, H) -> dict:
     return out
 
 
-def bench_batched(S, torch, scene, nviews: int) -> dict:
+def bench_batched(S, torch, scene, nviews: int, autotune: bool = True) -> dict:
     """64 cameras at resolutions from {320x180, 640x360, 1280x720, 1920x1080}, random
     yaw/pitch/zoom: one sp_render_batch call vs one sp_render_device per view."""
     views, sizes = S.random_views(scene, nviews, seed=3)
@@ -405,25 +405,31 @@ def bench_batched(S, torch, scene, nviews: int) -> dict:
     outs = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for (w, h) in sizes]
     ptrs = [o.data_ptr() for o in outs]
     bg = (0.0, 0.0, 0.0)
-    log(f"batched: {nviews} views, autotune")
-    tuned = r.autotune(views, sizes, bg, reps=3)
+    log(f"batched: {nviews} views" + (", autotune" if autotune else ""))
+    tuned = r.autotune(views, sizes, bg, reps=3) if autotune else r.tuning()
     r.render_batch(views, sizes, bg, ptrs, on_device=True)  # warm (captures)
     torch.cuda.synchronize()
-    reps = 3
-    t0 = time.perf_counter()
+    # Both forms: median of 5 repetitions (the sequential loop has host work
+    # between views, so its time varies more from run to run).
+    reps = 5
+    tbs = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         r.render_batch(views, sizes, bg, ptrs, on_device=True)
-    torch.cuda.synchronize()
-    tb = (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        tbs.append(time.perf_counter() - t0)
+    tb = float(np.median(tbs))
     for v, (w, h), p in zip(views, sizes, ptrs):
         r.render_device(v, w, h, bg, p)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    tss = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         for v, (w, h), p in zip(views, sizes, ptrs):
             r.render_device(v, w, h, bg, p)
-    torch.cuda.synchronize()
-    ts = (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        tss.append(time.perf_counter() - t0)
+    ts = float(np.median(tss))
     r.close()
     return {
         "workload": f"{nviews} views of the same scene, mixed resolutions, random pose/zoom (seed 3)",

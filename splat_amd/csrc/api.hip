@@ -504,7 +504,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
     const bool binned = mode != BL_DIFF;
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
-                            es, nullptr, nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs,
+                            es, nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs,
                             binned ? h->d_bsb : nullptr,
                             binned ? vsb : nullptr, h->d_pkey[0], h->d_pval[0], powner, h->pair_cap,
                             h->d_counters));

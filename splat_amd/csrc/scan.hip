@@ -53,6 +53,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict
     if (row_total && threadIdx.x == 0) row_total[blockIdx.x] = carry - base;
 }
 
+// Up to SCAN_MULTI arrays of `rows` rows each, scanned like k_scan_rows in
+// one launch: block b takes row b % rows of array b / rows.
+constexpr int SCAN_MULTI = 3;
+struct ScanArrays {
+    uint32_t *data[SCAN_MULTI];
+    uint32_t *row_total[SCAN_MULTI];
+};
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, uint64_t row_len, uint32_t rows) {
+    const uint32_t k = blockIdx.x / rows, r = blockIdx.x % rows;
+    uint32_t *row = a.data[k] + (uint64_t)r * row_len;
+    uint32_t carry = 0;
+    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) carry += scan_chunk(row, off, row_len, carry, true);
+    if (a.row_total[k] && threadIdx.x == 0) a.row_total[k][r] = carry;
+}
+
 // Radix digit rows: row r = (segment g, digit d) of `row_len` per-column
 // counts, of which the first radix_columns(min(seg_counts[g], n_cap),
 // tile_keys) are live.
@@ -88,6 +103,19 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
                             const uint32_t *row_base, uint32_t *row_total) {
     if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scan_rows, dim3(rows), dim3(SCAN_THREADS), 0, s, data, row_len, row_base, row_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
+                                  uint64_t row_len, uint32_t rows) {
+    if (rows == 0 || narrays <= 0) return hipSuccess;
+    if (narrays > SCAN_MULTI) return hipErrorInvalidValue;
+    ScanArrays a{};
+    for (int k = 0; k < narrays; ++k) {
+        a.data[k] = data[k];
+        a.row_total[k] = row_total[k];
+    }
+    hipLaunchKernelGGL(k_scan_rows_multi, dim3(rows * (uint32_t)narrays), dim3(SCAN_THREADS), 0, s, a, row_len, rows);
     return hipGetLastError();
 }
 

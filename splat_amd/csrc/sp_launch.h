@@ -14,6 +14,9 @@ namespace sp {
 // row); row_base[r] (optional) is added; row_total[r] (optional) gets the sum.
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total);
+// The same for up to 3 arrays of `rows` rows each, in one launch.
+hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
+                                  uint64_t row_len, uint32_t rows);
 // Radix tables: rows (segment g, digit d); only the live prefix of each row
 // (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries) is scanned,
 // in place and from 0; row_total[g * 256 + d] receives the row's sum.
@@ -57,7 +60,7 @@ uint32_t emit_blocks(uint32_t n);
 // view's depth-ordered splats, views in order, for emission phase `phase`
 // (EMIT_FIRST: the nearest K1 splats of each view, every tile; EMIT_SECOND:
 // the rest, only tiles with ip's row prefix counting them unfinished; vopen[v]
-// = 0 skips view v).
+// = 0 skips view v). FIRST clears vopen (when given) for the row prefixes.
 // bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
 // entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
 // (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
@@ -71,7 +74,7 @@ uint32_t emit_blocks(uint32_t n);
 // the tile pairs.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, const uint32_t *vopen, uint32_t *bcnt,
+                       EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters);
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW

@@ -42,6 +42,7 @@ namespace sp {
 namespace {
 
 constexpr uint32_t EXPAND_CHUNK = 4096;  // pairs per workgroup step of the expansion
+constexpr uint32_t EXPAND_CHUNK_BIN = 512;  // superblock pairs are ~20x fewer: smaller steps keep many CUs busy
 constexpr uint32_t EXPAND_GRID = 2048;   // workgroups of the expansion (grid-stride over chunks)
 
 struct Rect {
@@ -151,12 +152,13 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
                                                              uint32_t *__restrict__ bcnt,
                                                              uint32_t *__restrict__ bcnt_all,
                                                              uint32_t *__restrict__ bsb,
-                                                             const uint32_t *__restrict__ vopen) {
+                                                             uint32_t *__restrict__ vopen) {
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
     if (PHASE == EMIT_SECOND && vopen[v] == 0) {  // block-uniform: the first phase finished the view
         if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
         return;
     }
+    if (PHASE == EMIT_FIRST && vopen && local == 0 && threadIdx.x == 0) vopen[v] = 0;  // row prefixes add to it
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);  // dval holds n_splats entries per view
     const uint32_t k1 = first_count(nvis, split);
@@ -232,8 +234,9 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
         counters[CNT_STAGED] = 0u;  // the blends of this frame accumulate into it
     }
     const uint64_t nw = min(np, pair_cap);
-    for (uint64_t c0 = (uint64_t)blockIdx.x * EXPAND_CHUNK; c0 < nw; c0 += (uint64_t)gridDim.x * EXPAND_CHUNK) {
-        const uint64_t c1 = min(c0 + EXPAND_CHUNK, nw);
+    constexpr uint32_t CHUNK = BIN ? EXPAND_CHUNK_BIN : EXPAND_CHUNK;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * CHUNK; c0 < nw; c0 += (uint64_t)gridDim.x * CHUNK) {
+        const uint64_t c1 = min(c0 + CHUNK, nw);
         uint64_t q = c0;
         bool search = true;
         while (q < c1) {  // block-uniform
@@ -416,22 +419,15 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
 }
 
 // Inclusive per-row prefix counts of unfinished tiles: one wave per tile
-// row (lanes over 64 columns at a time), blockIdx.y = view. Block 0 of each
-// view also counts the view's unfinished tiles into vopen[v], so that the
-// second phase passes over views the first phase finished.
+// row (lanes over 64 columns at a time), blockIdx.y = view. Each row also
+// adds its count to vopen[v] (cleared by the first phase's count pass), so
+// that the second phase passes over views the first phase finished.
 __global__ __launch_bounds__(256) void k_row_prefix(const ViewParams *__restrict__ views,
                                                     const uint32_t *__restrict__ done, uint32_t *__restrict__ ip,
                                                     uint32_t *__restrict__ vopen) {
     const ViewParams &vp = views[blockIdx.y];
     const uint32_t tx_n = vp.ctx.tiles_x, ty_n = vp.ctx.tiles_y;
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    if (blockIdx.x == 0) {
-        uint32_t open = 0;
-        for (uint32_t t = threadIdx.x; t < tx_n * ty_n; t += 256) open += done[vp.tile_base + t] ? 0u : 1u;
-        uint32_t total;
-        block_exclusive_scan<256>(open, &total);
-        if (threadIdx.x == 0) vopen[blockIdx.y] = total;
-    }
     for (uint32_t ty = blockIdx.x * 4 + w; ty < ty_n; ty += gridDim.x * 4) {
         const uint32_t rb = vp.tile_base + ty * tx_n;
         uint32_t carry = 0;
@@ -442,6 +438,7 @@ __global__ __launch_bounds__(256) void k_row_prefix(const ViewParams *__restrict
             if (x < tx_n) ip[rb + x] = carry + inc;
             carry += (uint32_t)__shfl((int)inc, 63);
         }
+        if (lane == 0 && carry) atomicAdd(&vopen[blockIdx.y], carry);
     }
 }
 
@@ -695,7 +692,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                       EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
-                      uint32_t *bsb, uint32_t *vsb, const uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
+                      uint32_t *bsb, uint32_t *vsb, uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
                       uint32_t *powner, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     if (PHASE == EMIT_FIRST) {
@@ -708,17 +705,17 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                        vis_count, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
     hipError_t e = hipGetLastError();
-    // Per view: block offsets, and the view's pair total; then the view bases and the total.
-    if (e == hipSuccess) e = launch_scan_rows(s, bcnt, bpv, nviews, nullptr, vtot);
-    if (e == hipSuccess) e = launch_scan_rows(s, vtot, nviews, 1, nullptr, counters + count_slot);
-    // Unsplit pair count of every view (RenderStats.overlaps).
-    if (e == hipSuccess && PHASE != EMIT_SECOND) e = launch_scan_rows(s, bcnt_all, bpv, nviews, nullptr, view_pairs);
-    // Superblock pairs: block offsets, view bases and total.
-    if (e == hipSuccess && bin) e = launch_scan_rows(s, bsb, bpv, nviews, nullptr, vsb);
-    if (e == hipSuccess && bin) e = launch_scan_rows(s, vsb, nviews, 1, nullptr, counters + CNT_SBPAIRS);
+    // Per view: block offsets and the view's pair total (and, in FIRST, the
+    // unsplit pair count of every view, RenderStats.overlaps, and the binned
+    // superblock pairs); then the view bases and the totals.
+    uint32_t *rows_data[3] = {bcnt, bcnt_all, bsb}, *rows_tot[3] = {vtot, view_pairs, vsb};
+    const int nrow_arrays = PHASE == EMIT_SECOND ? 1 : (bin ? 3 : 2);
+    if (e == hipSuccess) e = launch_scan_rows_multi(s, nrow_arrays, rows_data, rows_tot, bpv, nviews);
+    uint32_t *view_data[2] = {vtot, vsb}, *view_tot[2] = {counters + count_slot, counters + CNT_SBPAIRS};
+    if (e == hipSuccess) e = launch_scan_rows_multi(s, bin ? 2 : 1, view_data, view_tot, nviews, 1);
     if (e != hipSuccess) return e;
     if (bin) {
-        const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
+        const uint64_t chunks = (pair_cap + EXPAND_CHUNK_BIN - 1) / EXPAND_CHUNK_BIN;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_dense<true>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, nullptr, pair_cap, counters);
@@ -740,7 +737,7 @@ uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THRE
 
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, const uint32_t *vopen, uint32_t *bcnt,
+                       EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
     if (phase == EMIT_SECOND)
@@ -748,7 +745,7 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, powner,
                                        pair_cap, CNT_PAIRS2, counters);
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
-                                  vtot, view_pairs, bsb, vsb, nullptr, pkey, pval, powner, pair_cap, CNT_PAIRS,
+                                  vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pair_cap, CNT_PAIRS,
                                   counters);
 }
 
