@@ -67,7 +67,6 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     const uint32_t px = qx0 + (lane_i & 7u), py = qy0 + (lane_i >> 3);
     const uint32_t lx = px - ox, ly = py - oy;
     const bool inside = px < v.w && py < v.h;
-    bool live = inside;
     const float fx = (float)ox;
     const float gx = (float)(lx & ~3u);  // 4 * group of the reference's 4-lane groups
     const float lane = (float)(lx & 3u);
@@ -80,8 +79,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         cr = st.y;
         cg = st.z;
         cb = st.w;
-        live = inside && !(T < T_MIN);
     }
+    // A pixel is live while inside the frame with T >= T_MIN (T only falls and
+    // is never NaN, so this is the reference's stop at T < T_MIN); deriving it
+    // from T keeps it a plain compare rather than a loop-carried mask.
+    const auto live = [&]() { return inside && T >= T_MIN; };
     uint32_t last = 0;
     bool finished = false;
     const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
@@ -102,11 +104,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
         // Records of the batch that can reach this wave's quadrant, 64 at a
         // time: one LDS read per lane and a ballot, then the set bits in order.
-        bool open = __any(live);
+        bool open = __any(live());
         for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
             unsigned long long reach = __ballot(g0 + lane_i < nb && ((s_qm[g0 + lane_i] >> w) & 1u));
             while (reach) {
-                if (!__any(live)) {  // every pixel of the quadrant saturated
+                if (!__any(live())) {  // every pixel of the quadrant saturated
                     open = false;
                     break;
                 }
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 reach &= reach - 1ull;
                 const uint4 r2 = s_rec[k][2];
                 const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-                const bool act = live && px >= bx0 && px < bx1 && py >= by0 && py < by1;
+                const bool act = live() && px >= bx0 && px < bx1 && py >= by0 && py < by1;
                 const uint4 r0 = s_rec[k][0];
                 const uint4 r1 = s_rec[k][1];
                 const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
@@ -139,11 +141,10 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                     cb = cb + __uint_as_float(r2.x) * wgt;
                     T = T * (1.0f - alpha);
                     if (MODE == BM_DIFF) last = (b - start) + k + 1;
-                    if (T < T_MIN) live = false;
                 }
             }
         }
-        if (__syncthreads_count(live) == 0) {
+        if (__syncthreads_count(live()) == 0) {
             b += BATCH;
             finished = true;
             break;
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     if (threadIdx.x == 0 && end > start) atomicAdd(&counters[CNT_STAGED], min(b, end) - start);
     if (MODE == BM_FIRST) {
         // `finished` is block-uniform, so the count below is reached by all threads or none.
-        if (!finished) finished = __syncthreads_count(live) == 0;
+        if (!finished) finished = __syncthreads_count(live()) == 0;
         if (!finished) {
             pstate[pix] = make_float4(T, cr, cg, cb);
             if (threadIdx.x == 0) done[gt] = 0u;

@@ -160,7 +160,9 @@ __device__ __forceinline__ uint32_t sat_u32(float v) {
 // unfused multiply-adds (self + a*b). Bit-identical to the CPU blend when
 // compiled without contraction.
 __device__ __forceinline__ float exp_neg(float xin) {
-    const float x = fmaxf(xin, -87.0f);
+    // max(xin, -87) as one median-of-three (fmaxf adds a canonicalising max
+    // in IEEE mode); equal for every non-NaN input, and inputs are finite.
+    const float x = __builtin_amdgcn_fmed3f(xin, -87.0f, 3.402823466e38f);
     const float n = rintf(x * 1.44269504088896340736f);
     const float f = (x - n * 0.6933594f) - n * -2.1219444e-4f;
     float p = 1.9875691e-4f;
