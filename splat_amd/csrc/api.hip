@@ -156,6 +156,7 @@ struct sp_handle {
     // Binned first phase: superblock list bounds, per-tile counts and row totals.
     uint32_t *d_sb_start = nullptr, *d_tcount = nullptr, *d_trow = nullptr;
     uint32_t *d_segc = nullptr;  // (sb_cap - 1) x BIN_SEGS x 16 segment counts
+    uint32_t *d_sbtile = nullptr;  // (sb_cap - 1) x 16 global tile ids
     uint64_t sb_cap = 0, tcount_cap = 0;
     uint8_t *d_rgb = nullptr;
     uint64_t rgb_cap = 0;
@@ -356,9 +357,11 @@ int ensure_bin(sp_handle *h, uint64_t tiles, uint64_t sbs) {
         const uint64_t seg_words = (uint64_t)sp::BIN_SEGS * sp::SB * sp::SB;
         dfree(h, h->d_sb_start, h->sb_cap * 4);
         dfree(h, h->d_segc, (h->sb_cap ? h->sb_cap - 1 : 0) * seg_words * 4);
+        dfree(h, h->d_sbtile, (h->sb_cap ? h->sb_cap - 1 : 0) * sp::SB * sp::SB * 4);
         h->sb_cap = 0;
         int rc = dalloc(h, (void **)&h->d_sb_start, (sbs + 1) * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_segc, sbs * seg_words * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_sbtile, sbs * sp::SB * sp::SB * 4);
         if (rc) return rc;
         h->sb_cap = sbs + 1;
     }
@@ -520,7 +523,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         pi = 1 - si;
         // Tile ids are kept only when the lists are read back (no blend).
         HIP_TRY(sp::launch_bin_lists(s, h->d_views, nviews, total_sbs, total_tiles, h->d_sb_start, h->d_pkey[si],
-                                     h->d_pval[si], h->d_segc, h->d_tcount, h->d_trow, h->d_tile_start,
+                                     h->d_pval[si], h->d_segc, h->d_sbtile, h->d_tcount, h->d_trow,
+                                     h->d_tile_start,
                                      mode == BL_NONE ? h->d_pkey[pi] : nullptr, h->d_pval[pi], h->pair_cap,
                                      h->d_counters));
     } else {
@@ -821,6 +825,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_tile_start);
     (void)hipFree(h->d_sb_start);
     (void)hipFree(h->d_segc);
+    (void)hipFree(h->d_sbtile);
     (void)hipFree(h->d_tcount);
     (void)hipFree(h->d_trow);
     (void)hipFree(h->d_bsb);

@@ -16,7 +16,7 @@ constexpr uint32_t TILE = 16;
 // Superblock: SB x SB tiles, the bin of the first emission phase (tiles.hip).
 constexpr uint32_t SB = 4;
 constexpr uint32_t BIN_SCAN_ROW = 2048;  // tiles per row of the binned tile-count scan
-constexpr uint32_t BIN_SEGS = 16;        // workgroups per superblock list in the binned first phase
+constexpr uint32_t BIN_SEGS = 4;         // workgroups per superblock list in the binned first phase
 constexpr uint32_t SB_SUB_SHIFT = 24;    // superblock pair keys: superblock id below, sub-rectangle above
 
 // Workgroup shapes.

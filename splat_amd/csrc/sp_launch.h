@@ -84,10 +84,10 @@ uint32_t bin_scan_rows(uint32_t total_tiles);
 // | sub-rectangle << SB_SUB_SHIFT, sval = record index; ranges
 // sb_start[0..total_sbs]), the tile lists in tile order (pval = record index,
 // pkey = tile id when pkey is given) and tile_start[0..total_tiles]. segc:
-// total_sbs * BIN_SEGS * SB * SB scratch counts.
+// total_sbs * BIN_SEGS * SB * SB scratch counts; sbtile: total_sbs * SB * SB.
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
                             uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *skey,
-                            const uint32_t *sval, uint32_t *segc, uint32_t *tcount, uint32_t *trow,
+                            const uint32_t *sval, uint32_t *segc, uint32_t *sbtile, uint32_t *tcount, uint32_t *trow,
                             uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters);
 // Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every

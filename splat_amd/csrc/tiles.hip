@@ -42,7 +42,9 @@ namespace sp {
 namespace {
 
 constexpr uint32_t EXPAND_CHUNK = 4096;  // pairs per workgroup step of the expansion
-constexpr uint32_t EXPAND_CHUNK_BIN = 512;  // superblock pairs are ~20x fewer: smaller steps keep many CUs busy
+// Superblock pairs are ~20x fewer than tile pairs: their chunk follows the
+// live count (512 to 4096 pairs), so few pairs still spread over many CUs.
+constexpr uint32_t EXPAND_CHUNK_BIN_MIN = 512;
 constexpr uint32_t EXPAND_GRID = 2048;   // workgroups of the expansion (grid-stride over chunks)
 
 struct Rect {
@@ -234,7 +236,10 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
         counters[CNT_STAGED] = 0u;  // the blends of this frame accumulate into it
     }
     const uint64_t nw = min(np, pair_cap);
-    constexpr uint32_t CHUNK = BIN ? EXPAND_CHUNK_BIN : EXPAND_CHUNK;
+    const uint32_t CHUNK =
+        BIN ? (uint32_t)min((uint64_t)EXPAND_CHUNK,
+                            max((uint64_t)EXPAND_CHUNK_BIN_MIN, ((nw / gridDim.x + 255) / 256) * 256))
+            : EXPAND_CHUNK;
     for (uint64_t c0 = (uint64_t)blockIdx.x * CHUNK; c0 < nw; c0 += (uint64_t)gridDim.x * CHUNK) {
         const uint64_t c1 = min(c0 + CHUNK, nw);
         uint64_t q = c0;
@@ -485,7 +490,7 @@ __device__ __forceinline__ SbGeom sb_geom(const ViewParams *views, uint32_t nvie
 
 // A superblock's list is cut into BIN_SEGS segments (each a whole number of
 // 256-entry chunks, at least one), one workgroup per (superblock, segment),
-// so that long lists (the centre of a view) spread over many CUs.
+// so that long lists (the centre of a view) spread over several CUs.
 __device__ __forceinline__ void bin_segment(uint32_t e0, uint32_t e1, uint32_t y, uint32_t &s0, uint32_t &s1) {
     const uint32_t len = e1 - e0;
     const uint32_t per = max(256u, (((len + BIN_SEGS - 1) / BIN_SEGS) + 255u) & ~255u);
@@ -521,12 +526,13 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ 
 }
 
 // One thread per (superblock, tile): segc becomes the exclusive prefix over
-// the superblock's segments, tcount[gt] the tile's list length; the padding
-// tcount[total_tiles, tcount_len) is cleared for the row scan.
+// the superblock's segments, tcount[gt] the tile's list length and
+// sbtile[sb * 16 + b] the global tile id (~0 past the frame's edge); the
+// padding tcount[total_tiles, tcount_len) is cleared for the row scan.
 __global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict__ views, uint32_t nviews,
                                                     uint32_t total_sbs, uint32_t *__restrict__ segc,
-                                                    uint32_t *__restrict__ tcount, uint32_t total_tiles,
-                                                    uint32_t tcount_len) {
+                                                    uint32_t *__restrict__ tcount, uint32_t *__restrict__ sbtile,
+                                                    uint32_t total_tiles, uint32_t tcount_len) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < (uint64_t)total_sbs * (SB * SB)) {
         const uint32_t sb = (uint32_t)(g / (SB * SB)), b = (uint32_t)(g % (SB * SB));
@@ -539,7 +545,12 @@ __global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict
         }
         const SbGeom q = sb_geom(views, nviews, sb);
         const uint32_t tx = q.bx + b % SB, ty = q.by + b / SB;
-        if (tx < q.tiles_x && ty < q.tiles_y) tcount[q.tile_base + ty * q.tiles_x + tx] = acc;
+        uint32_t gt = ~0u;
+        if (tx < q.tiles_x && ty < q.tiles_y) {
+            gt = q.tile_base + ty * q.tiles_x + tx;
+            tcount[gt] = acc;
+        }
+        sbtile[g] = gt;
     }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t t = total_tiles + g; t < tcount_len; t += stride) tcount[t] = 0;
@@ -551,7 +562,7 @@ __global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict
 // given) = tile id. tscan: in-row exclusive prefix of tcount (rows of
 // BIN_SCAN_ROW tiles); trow: exclusive prefix of the row totals; segc: the
 // segment offsets within each tile's list.
-__global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict__ views, uint32_t nviews,
+__global__ __launch_bounds__(256) void k_bin_write(const uint32_t *__restrict__ sbtile,
                                                    const uint32_t *__restrict__ sb_start,
                                                    const uint32_t *__restrict__ skey,
                                                    const uint32_t *__restrict__ sval,
@@ -561,58 +572,71 @@ __global__ __launch_bounds__(256) void k_bin_write(const ViewParams *__restrict_
                                                    uint32_t *__restrict__ tile_start, uint32_t *__restrict__ pkey,
                                                    uint32_t *__restrict__ pval, uint64_t pair_cap,
                                                    const uint32_t *__restrict__ counters) {
-    __shared__ uint32_t s_next[SB * SB];  // next list position of each tile
-    __shared__ uint32_t s_wc[4][SB * SB];
+    __shared__ uint32_t s_next[SB * SB];     // next list position of each tile
+    __shared__ uint32_t s_gt[SB * SB];       // global tile id of each tile (~0: past the frame's edge)
+    __shared__ uint32_t s_wc[4][SB * SB];    // this chunk's entries per (wave, tile)
+    __shared__ uint32_t s_base[4][SB * SB];  // first position per (wave, tile) in this chunk
+    __shared__ uint32_t s_tot[SB * SB];      // this chunk's entries per tile
     const uint32_t sb = blockIdx.x, y = blockIdx.y, w = threadIdx.x >> 6, lane = lane_id();
     uint32_t s0, s1;
     bin_segment(sb_start[sb], sb_start[sb + 1], y, s0, s1);
     if (sb == 0 && y == 0 && threadIdx.x == 0)
         tile_start[total_tiles] = (uint32_t)min((uint64_t)counters[CNT_PAIRS], pair_cap);
     if (s0 >= s1 && y > 0) return;  // block-uniform; segment 0 always writes the tile starts
-    const SbGeom g = sb_geom(views, nviews, sb);
+    // The first chunk's entries load while the tile starts are set up.
+    uint32_t key_n = 0, rid_n = 0;
+    if (s0 + threadIdx.x < s1) {
+        key_n = skey[s0 + threadIdx.x];
+        rid_n = sval[s0 + threadIdx.x];
+    }
     if (threadIdx.x < SB * SB) {
-        const uint32_t tx = g.bx + (threadIdx.x % SB), ty = g.by + (threadIdx.x / SB);
+        const uint32_t gt = sbtile[(uint64_t)sb * (SB * SB) + threadIdx.x];
         uint32_t start = 0;
-        if (tx < g.tiles_x && ty < g.tiles_y) {
-            const uint32_t gt = g.tile_base + ty * g.tiles_x + tx;
+        if (gt != ~0u) {
             start = trow[gt / BIN_SCAN_ROW] + tscan[gt];
             if (y == 0) tile_start[gt] = (uint32_t)min((uint64_t)start, pair_cap);
         }
+        s_gt[threadIdx.x] = gt;
         s_next[threadIdx.x] = start + segc[((uint64_t)sb * BIN_SEGS + y) * (SB * SB) + threadIdx.x];
     }
-    __syncthreads();
     for (uint32_t c = s0; c < s1; c += 256) {  // block-uniform
         const uint32_t e = c + threadIdx.x;
-        uint32_t m = 0, rid = 0;
-        if (e < s1) {
-            m = sb_sub_mask(skey[e] >> SB_SUB_SHIFT);
-            rid = sval[e];
+        const uint32_t m = e < s1 ? sb_sub_mask(key_n >> SB_SUB_SHIFT) : 0u, rid = rid_n;
+        if (e + 256 < s1) {  // next chunk
+            key_n = skey[e + 256];
+            rid_n = sval[e + 256];
         }
         unsigned long long bal[SB * SB];
+        uint32_t cnt = 0;  // lane b < 16: this wave's entries covering tile b
 #pragma unroll
         for (uint32_t b = 0; b < SB * SB; ++b) {
             bal[b] = __ballot((m >> b) & 1u);
-            if (lane == 0) s_wc[w][b] = (uint32_t)__popcll(bal[b]);
+            cnt = lane == b ? (uint32_t)__popcll(bal[b]) : cnt;
         }
-        __syncthreads();
+        if (lane < SB * SB) s_wc[w][lane] = cnt;
+        __syncthreads();  // s_wc complete; s_next holds this chunk's starts
+        if (threadIdx.x < 4 * SB * SB) {
+            const uint32_t ww = threadIdx.x / (SB * SB), b = threadIdx.x % (SB * SB);
+            uint32_t base = s_next[b];
+            for (uint32_t k = 0; k < ww; ++k) base += s_wc[k][b];
+            s_base[ww][b] = base;
+            if (ww == 0) s_tot[b] = s_wc[0][b] + s_wc[1][b] + s_wc[2][b] + s_wc[3][b];
+        }
+        __syncthreads();  // s_base complete
+        const uint32_t basev = s_base[w][lane % (SB * SB)];
+        if (threadIdx.x < SB * SB) s_next[threadIdx.x] += s_tot[threadIdx.x];  // read again after the next first barrier
 #pragma unroll
         for (uint32_t b = 0; b < SB * SB; ++b) {
+            const uint32_t base_b = __builtin_amdgcn_readlane(basev, b);
             if ((m >> b) & 1u) {
-                uint32_t pos = s_next[b] +
-                               __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[b] >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal[b], 0u));
-                for (uint32_t ww = 0; ww < w; ++ww) pos += s_wc[ww][b];
+                const uint32_t pos = base_b + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[b] >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal[b], 0u));
                 if (pos < pair_cap) {
                     pval[pos] = rid;
-                    if (pkey) pkey[pos] = g.tile_base + (g.by + b / SB) * g.tiles_x + g.bx + b % SB;
+                    if (pkey) pkey[pos] = s_gt[b];
                 }
             }
         }
-        __syncthreads();
-        if (threadIdx.x < SB * SB)
-            s_next[threadIdx.x] +=
-                s_wc[0][threadIdx.x] + s_wc[1][threadIdx.x] + s_wc[2][threadIdx.x] + s_wc[3][threadIdx.x];
-        __syncthreads();
     }
 }
 
@@ -715,7 +739,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     if (e == hipSuccess) e = launch_scan_rows_multi(s, bin ? 2 : 1, view_data, view_tot, nviews, 1);
     if (e != hipSuccess) return e;
     if (bin) {
-        const uint64_t chunks = (pair_cap + EXPAND_CHUNK_BIN - 1) / EXPAND_CHUNK_BIN;
+        const uint64_t chunks = (pair_cap + EXPAND_CHUNK_BIN_MIN - 1) / EXPAND_CHUNK_BIN_MIN;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_dense<true>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, nullptr, pair_cap, counters);
@@ -751,7 +775,7 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
 
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
                             uint32_t total_tiles, const uint32_t *sb_start, const uint32_t *skey,
-                            const uint32_t *sval, uint32_t *segc, uint32_t *tcount, uint32_t *trow,
+                            const uint32_t *sval, uint32_t *segc, uint32_t *sbtile, uint32_t *tcount, uint32_t *trow,
                             uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters) {
     if (total_tiles == 0 || total_sbs == 0) return launch_fill(s, tile_start, 1, 0u);
@@ -762,14 +786,14 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
     if (e != hipSuccess) return e;
     const uint64_t pthreads = std::max<uint64_t>((uint64_t)total_sbs * SB * SB, (uint64_t)rows * BIN_SCAN_ROW - total_tiles);
     const uint32_t pgrid = (uint32_t)std::min<uint64_t>((pthreads + 255) / 256, 65535);
-    hipLaunchKernelGGL(k_bin_prefix, dim3(pgrid), dim3(256), 0, s, views, nviews, total_sbs, segc, tcount,
+    hipLaunchKernelGGL(k_bin_prefix, dim3(pgrid), dim3(256), 0, s, views, nviews, total_sbs, segc, tcount, sbtile,
                        total_tiles, rows * BIN_SCAN_ROW);
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_scan_rows(s, tcount, BIN_SCAN_ROW, rows, nullptr, trow);
     if (e == hipSuccess) e = launch_scan_rows(s, trow, rows, 1, nullptr, nullptr);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, views, nviews, sb_start, skey, sval, segc, tcount,
-                       trow, total_tiles, tile_start, pkey, pval, pair_cap, counters);
+    hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, sbtile, sb_start, skey, sval, segc, tcount, trow,
+                       total_tiles, tile_start, pkey, pval, pair_cap, counters);
     return hipGetLastError();
 }
 
