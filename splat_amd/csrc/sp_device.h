@@ -63,6 +63,41 @@ __device__ __forceinline__ uint32_t radix_columns(uint64_t n, uint32_t tile_keys
     return (t + g - 1) / g;
 }
 
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Sum of v over the 64 lanes, left in lane 63 (the same DPP steps as
+// wave_total_f). Every lane must be active.
+__device__ __forceinline__ uint32_t wave_total_u(uint32_t v) {
+    v += dpp_u<0xB1, 0xF>(v);
+    v += dpp_u<0x4E, 0xF>(v);
+    v += dpp_u<0x141, 0xF>(v);
+    v += dpp_u<0x140, 0xF>(v);
+    v += dpp_u<0x142, 0xA>(v);
+    v += dpp_u<0x143, 0xC>(v);
+    return v;
+}
+
+// Totals of N values over a 256-thread block, one barrier round; every
+// thread of the block calls it with all lanes active, and every thread gets
+// the totals.
+template <int N>
+__device__ __forceinline__ void block_totals(const uint32_t (&v)[N], uint32_t (&tot)[N]) {
+    __shared__ uint32_t s_t[4][N];
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const uint32_t t = wave_total_u(v[k]);
+        if (lane_id() == 63) s_t[w][k] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) tot[k] = (s_t[0][k] + s_t[1][k]) + (s_t[2][k] + s_t[3][k]);
+    __syncthreads();  // s_t is reused by the next call
+}
+
 // The view of a batch that owns global tile gt: the last view whose
 // tile_base <= gt (bases ascend with the view index).
 __device__ __forceinline__ uint32_t view_of_tile(const ViewParams *views, uint32_t nviews, uint32_t gt) {

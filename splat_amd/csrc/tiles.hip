@@ -53,13 +53,15 @@ struct Rect {
 };
 
 // Tile rectangle of the j-th depth-ordered splat of view v, from the
-// depth-ordered pixel boxes (box[vbase + j] = the record's packed x0|y0, x1|y1).
+// depth-ordered pixel boxes (box[vbase + j] = the record's packed x0|y0, x1|y1);
+// with WITH_ID, also its record index (the count passes do not need it).
+template <bool WITH_ID = true>
 __device__ __forceinline__ Rect splat_rect(const uint32_t *__restrict__ dval, const uint2 *__restrict__ box,
                                            uint64_t vbase, uint32_t j, uint32_t nvis, uint32_t tiles_x,
                                            uint32_t tile_base) {
     Rect r{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
     if (j < nvis) {
-        r.id = (uint32_t)(vbase + dval[vbase + j]);
+        if (WITH_ID) r.id = (uint32_t)(vbase + dval[vbase + j]);
         const uint2 b = box[vbase + j];
         const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
         if (x0 < x1 && y0 < y1) {
@@ -166,19 +168,24 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
     const uint32_t k1 = first_count(nvis, split);
     const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
     const uint32_t j = local * EMIT_THREADS + threadIdx.x;
-    const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
-    uint32_t total;
-    block_exclusive_scan<EMIT_THREADS>(phase_count<PHASE>(r, j, k1, ip, tile_base, tiles_x), &total);
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = total;
-    if (PHASE != EMIT_SECOND) {
-        uint32_t all;
-        block_exclusive_scan<EMIT_THREADS>(r.cnt, &all);
-        if (threadIdx.x == 0) bcnt_all[blockIdx.x] = all;
-        if (bsb) {  // binned first phase: superblock pairs of the same splats
-            const SbRect q = sb_rect(r, (tiles_x + SB - 1) / SB, views[v].sb_base);
-            uint32_t sbt;
-            block_exclusive_scan<EMIT_THREADS>(j < k1 ? q.cnt : 0u, &sbt);
-            if (threadIdx.x == 0) bsb[blockIdx.x] = sbt;
+    const Rect r = splat_rect<false>(dval, box, vbase, j, nvis, tiles_x, tile_base);
+    const uint32_t c = phase_count<PHASE>(r, j, k1, ip, tile_base, tiles_x);
+    if (PHASE == EMIT_SECOND) {
+        const uint32_t v1[1] = {c};
+        uint32_t t1[1];
+        block_totals<1>(v1, t1);
+        if (threadIdx.x == 0) bcnt[blockIdx.x] = t1[0];
+    } else {
+        // Tile pairs of this phase, unsplit pairs (RenderStats.overlaps) and,
+        // binned, superblock pairs of the phase's splats.
+        const uint32_t sbc = bsb && j < k1 ? sb_rect(r, (tiles_x + SB - 1) / SB, views[v].sb_base).cnt : 0u;
+        const uint32_t v3[3] = {c, r.cnt, sbc};
+        uint32_t t3[3];
+        block_totals<3>(v3, t3);
+        if (threadIdx.x == 0) {
+            bcnt[blockIdx.x] = t3[0];
+            bcnt_all[blockIdx.x] = t3[1];
+            if (bsb) bsb[blockIdx.x] = t3[2];
         }
     }
 }
@@ -508,9 +515,11 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t *__restrict__ 
     bin_segment(sb_start[sb], sb_start[sb + 1], y, s0, s1);
     uint32_t mine = 0;  // lane b < 16: this wave's entries covering tile b
     if (s0 < s1) {
+        uint32_t key_n = s0 + threadIdx.x < s1 ? skey[s0 + threadIdx.x] : 0u;
         for (uint32_t c = s0; c < s1; c += 256) {  // block-uniform
             const uint32_t e = c + threadIdx.x;
-            const uint32_t m = e < s1 ? sb_sub_mask(skey[e] >> SB_SUB_SHIFT) : 0u;
+            const uint32_t m = e < s1 ? sb_sub_mask(key_n >> SB_SUB_SHIFT) : 0u;
+            if (e + 256 < s1) key_n = skey[e + 256];  // next chunk
 #pragma unroll
             for (uint32_t b = 0; b < SB * SB; ++b) {
                 const uint32_t k = (uint32_t)__popcll(__ballot((m >> b) & 1u));
