@@ -11,12 +11,15 @@
 // search over in-block offsets. The dense expansion deals fixed-size chunks
 // of pairs to workgroups (the nearest splats cover thousands of tiles, so
 // per-splat-block work would be badly unbalanced); the sparse second phase
-// keeps one workgroup per splat block. Pairs come out view by view in (depth, scene index)
-// order, carrying a global tile id (view tile base + tile) and a record index
-// (view * n + scene index), ready for the stable tile-id sort. The
-// differentiable path sorts the emission index instead and keeps the record
-// index per emission index (powner), so that per-pair gradients can be
-// written back in emission order, where each splat's pairs are contiguous.
+// keeps one workgroup per splat block. Pairs come out view by view in
+// (depth, scene index) order, carrying a global tile id (view tile base +
+// tile) and a record index (view * n + scene index), ready for the stable
+// tile-id sort. The differentiable path sorts the emission index instead and
+// keeps the record index per emission index (powner), so that per-pair
+// gradients can be written back in emission order, where each splat's pairs
+// are contiguous. The plain path's first phase is binned instead of sorted
+// (superblock pairs, then per-superblock tile lists; see "Binned first
+// phase" below).
 //
 // Front-to-back split. Almost every tile saturates (all pixels reach
 // T < 1e-4) within the first few percent of its depth-ordered list, so the
@@ -31,8 +34,8 @@
 // (RenderStats.overlaps).
 //
 // Ranges: bucket bounds tile_start[t] (src/render/cpu.rs:150-163) from the
-// boundaries of the tile-sorted keys; empty tiles between two boundaries get
-// the same start.
+// tile-sorted keys (binary search per tile for long lists, key boundaries
+// otherwise); empty tiles between two boundaries get the same start.
 #include <algorithm>
 
 #include "sp_device.h"
