@@ -119,26 +119,25 @@ def main() -> None:
     torch.cuda.synchronize()
 
     log(f"timed: {args.steps} views")
-    stats = []
-    infos = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = r.render_device(views[i % len(views)], W, H, bg, out.data_ptr())
-        stats.append(st)
-        infos.append(r.frame_info())
+        r.render_device(views[i % len(views)], W, H, bg, out.data_ptr())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = args.steps / elapsed
     mean = lambda xs: float(np.mean(xs)) if xs else 0.0
 
-    # Stage breakdown: a second handle with timing events between the stages
-    # (same tuning), over the first views of the orbit.
+    # Stage breakdown and work counters: a second handle with timing events
+    # between the stages (same tuning), over the first views of the orbit.
     log("stage timing")
     rt = S.HipRenderer(scene)
     rt.set_tuning(**{k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")})
     rt.render_device(views[0], W, H, bg, out.data_ptr())
-    stats = [rt.render_device(views[i % len(views)], W, H, bg, out.data_ptr()) for i in range(min(args.steps, 8))]
+    stats, infos = [], []
+    for i in range(min(args.steps, 8)):
+        stats.append(rt.render_device(views[i % len(views)], W, H, bg, out.data_ptr()))
+        infos.append(rt.frame_info())
     rt.close()
     proj_ms = mean([s.project_ms for s in stats])
     sort_ms = mean([s.sort_ms for s in stats])

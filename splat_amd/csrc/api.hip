@@ -1140,7 +1140,7 @@ int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t
         }
         return SP_OK;
     };
-    for (uint32_t sh : {0u, 2u, 3u, 4u, 5u}) {
+    for (uint32_t sh : {0u, 1u, 2u, 3u, 4u, 5u}) {
         if (sh == cur.split_shift) continue;
         sp_tuning t = cur;
         t.split_shift = sh;
