@@ -18,7 +18,9 @@
 //        dL/dmean2 = dL/dpw * (A dx + B dy, B dx + C dy)
 //        dL/dconic = dL/dpw * (-dx^2 / 2, -dx dy, -dy^2 / 2)
 //      with G = dL/dpixel. The nine values are summed over the tile's 256
-//      pixels in a fixed order (DPP sums within a wave, then the four waves) and
+//      pixels in a fixed order (DPP sums within a wave, then the four waves);
+//      the mean and conic terms are summed as moments of dL/dpw (dx, dy,
+//      dx^2, dx dy, dy^2) and multiplied by the conic once per record. They are
 //      stored per pair at its emission index. Records whose falloff cannot
 //      reach a quadrant are never visited by its wave (one ballot over the
 //      staged records' reach bits).
@@ -142,26 +144,31 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             for (int c = 0; c < NG2D; ++c) gv[c] = 0.0f;
             if (con) {
                 const float om = 1.0f - alpha;
-                const float Ti = T / om;
+                // One hardware reciprocal (1 ulp) for both divisions: gradients
+                // need no bit-exactness, and T's recovery drifts by ulps only.
+                const float rom = __builtin_amdgcn_rcpf(om);
+                const float Ti = T * rom;
                 const float c0 = __uint_as_float(r1.z), c1 = __uint_as_float(r1.w), c2 = __uint_as_float(r2.x);
                 const float aT = alpha * Ti;
                 gv[6] = G0 * aT;
                 gv[7] = G1 * aT;
                 gv[8] = G2 * aT;
-                const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) / om;
+                const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) * rom;
                 S0 += c0 * aT;
                 S1 += c1 * aT;
                 S2 += c2 * aT;
                 T = Ti;
                 if (raw < ALPHA_MAX) {
                     gv[5] = ga * ex;
+                    // Moments of dL/dpw over the pixels; the record's conic
+                    // coefficients are applied once per record after the sums.
                     const float gpw = ga * alpha;
-                    const float A = -2.0f * na, B = -nb2, C = -2.0f * nc;
-                    gv[0] = gpw * (A * dx + B * dy);
-                    gv[1] = gpw * (B * dx + C * dy);
-                    gv[2] = -0.5f * gpw * (dx * dx);
-                    gv[3] = -gpw * (dx * dy);
-                    gv[4] = -0.5f * gpw * (dy * dy);
+                    const float u = gpw * dx, wdy = gpw * dy;
+                    gv[0] = u;
+                    gv[1] = wdy;
+                    gv[2] = u * dx;
+                    gv[3] = u * dy;
+                    gv[4] = wdy * dy;
                 }
             }
             if (__any(con)) {  // wave-uniform, so all 64 lanes take part in the DPP sums
@@ -175,10 +182,25 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         __syncthreads();
         if (threadIdx.x < nb) {
             const uint32_t e = s_e[threadIdx.x];
+            float sum[NG2D];
 #pragma unroll
             for (int c = 0; c < NG2D; ++c)
-                gpair[c * gstride + e] = (s_part[0][c][threadIdx.x] + s_part[1][c][threadIdx.x]) +
-                                         (s_part[2][c][threadIdx.x] + s_part[3][c][threadIdx.x]);
+                sum[c] = (s_part[0][c][threadIdx.x] + s_part[1][c][threadIdx.x]) +
+                         (s_part[2][c][threadIdx.x] + s_part[3][c][threadIdx.x]);
+            // dL/dmean = gpw (A dx + B dy, B dx + C dy) and dL/dconic =
+            // gpw (-dx^2/2, -dx dy, -dy^2/2), from the moment sums
+            // (A = -2 na, B = -nb2, C = -2 nc: the conic a, b, c).
+            const float A = -2.0f * __uint_as_float(s_rec[threadIdx.x][0].z);
+            const float B = -__uint_as_float(s_rec[threadIdx.x][0].w);
+            const float C = -2.0f * __uint_as_float(s_rec[threadIdx.x][1].x);
+            const float U = sum[0], W = sum[1];
+            sum[0] = A * U + B * W;
+            sum[1] = B * U + C * W;
+            sum[2] = -0.5f * sum[2];
+            sum[3] = -sum[3];
+            sum[4] = -0.5f * sum[4];
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c) gpair[c * gstride + e] = sum[c];
             stamp[e] = stamp_id;  // this call wrote gpair[.., e]
         }
         __syncthreads();
