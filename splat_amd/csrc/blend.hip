@@ -94,11 +94,15 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
             uint32_t id = sval[idx];
             if (MODE == BM_DIFF) id = powner[id];
             const uint4 *r = rec + 3 * (uint64_t)id;
-            const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
+            uint4 r0 = r[0];
+            const uint4 r1 = r[1], r2 = r[2];
+            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy);
+            // The staged x is the tile's ((ox + 0.5) - x), the reference's
+            // per-record term of every pixel offset, computed once here.
+            r0.x = __float_as_uint((fx + 0.5f) - __uint_as_float(r0.x));
             s_rec[threadIdx.x][0] = r0;
             s_rec[threadIdx.x][1] = r1;
             s_rec[threadIdx.x][2] = r2;
-            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy);
         }
         __syncthreads();
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
@@ -119,10 +123,9 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 const bool act = live() && px >= bx0 && px < bx1 && py >= by0 && py < by1;
                 const uint4 r0 = s_rec[k][0];
                 const uint4 r1 = s_rec[k][1];
-                const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
+                const float cx0 = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
                 const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w);
                 const float nc = __uint_as_float(r1.x);
-                const float cx0 = (fx + 0.5f) - sx;
                 const float dx = lane + (gx + cx0);
                 const float dy = pyc - sy;
                 const float qx = (dx * dx) * na;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 // reference masks them); a wave with none left skips the rest.
                 if (!__any(act && pw >= __uint_as_float(r2.w))) continue;
                 const float op = __uint_as_float(r1.y);
-                const float alpha = fminf(exp_neg(pw) * op, ALPHA_MAX);
+                const float alpha = fminf(exp_neg_times(pw, op), ALPHA_MAX);
                 if (act && pw <= 0.0f && alpha >= ALPHA_MIN) {
                     const float wgt = alpha * T;
                     cr = cr + __uint_as_float(r1.z) * wgt;

@@ -205,6 +205,27 @@ __device__ __forceinline__ float exp_neg(float xin) {
     return r * __int_as_float(((int)n + 127) << 23);
 }
 
+// exp_neg(x) * op, equal to the reference's product wherever it is a normal
+// float: exp_neg's result r * 2^n is exact (n >= -126, and r >= 1 when
+// n = -126, for clamped inputs), so scaling the rounded r * op by 2^n rounds
+// the same value once.
+// (A subnormal product is below ALPHA_MIN, where the blend masks the pixel
+// either way.) One v_ldexp_f32 replaces building 2^n and one multiply.
+__device__ __forceinline__ float exp_neg_times(float xin, float op) {
+    const float x = __builtin_amdgcn_fmed3f(xin, -87.0f, 3.402823466e38f);
+    const float n = rintf(x * 1.44269504088896340736f);
+    const float f = (x - n * 0.6933594f) - n * -2.1219444e-4f;
+    float p = 1.9875691e-4f;
+    p = 1.3981999e-3f + p * f;
+    p = 8.333452e-3f + p * f;
+    p = 4.1665796e-2f + p * f;
+    p = 1.6666665e-1f + p * f;
+    p = 0.5f + p * f;
+    const float f2 = f * f;
+    const float r = (f + 1.0f) + p * f2;
+    return __builtin_amdgcn_ldexpf(r * op, (int)n);
+}
+
 // to_u8 (src/render/mod.rs:282-285), one channel.
 __device__ __forceinline__ uint32_t to_u8(float v) {
     if (v < 0.0f) v = 0.0f;
