@@ -15,6 +15,9 @@ Also reported beside the headline value:
     bounded sample of the same views, timed on this host's cores.
   * precision: 3,000,000 splats stored as f32 / FP16 / BF16: views/s and each
     stage's error against the FP64 oracle (conics, colours, final image).
+  * scene files: the same scene written as a 3DGS PLY and as .splat, read back
+    by the native loader (parse + finalize) and uploaded; load rate per thread
+    count and the render of the loaded scene against the original.
   * differentiation: 500,000 splats, 8 views at 1280x720, L2 loss against
     targets rendered from parameters perturbed by N(0, 0.01): forward and
     backward wall times, and the gradient checked against the FP64 reference
@@ -70,6 +73,7 @@ def main() -> None:
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--diff-splats", type=int, default=500_000)
     ap.add_argument("--gradcheck-params", type=int, default=256)
+    ap.add_argument("--no-io", action="store_true", help="skip the scene-file leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 5 times (reference kernel for HBM byte counters)")
@@ -223,6 +227,11 @@ def main() -> None:
             "achieved": round(tf, 2), "peak": F32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tf / F32_PEAK_TFLOPS, 4), "view": 0,
         }
+    if not args.no_io:
+        try:
+            result["scene_files"] = bench_scene_io(S, torch, scene, views[0], W, H, r)
+        except Exception as e:
+            result["scene_files"] = {"error": repr(e)}
     r.close()
     del r, scene
     if not args.no_diff:
@@ -438,6 +447,56 @@ def bench_batched(S, torch, scene, nviews: int, autotune: bool = True) -> dict:
         "pixels": int(sum(w * h for w, h in sizes)),
         "tuning": {k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")},
     }
+
+
+def bench_scene_io(S, torch, scene, view, W, H, r) -> dict:
+    """Scene::load of the headline scene from a 3DGS PLY (62 f32 per splat) and from
+    .splat: wall time of the native reader (parse + finalize, warm page cache) on all
+    host threads (median of 3) and on one thread, then the upload (renderer
+    construction). The loaded scene renders against the original at PSNR; the PLY
+    round trip is exact except where exp(ln(scale)) and sigmoid(logit(opacity))
+    round to a neighbouring f32."""
+    import tempfile
+
+    from splat_amd import scene as SC
+
+    cores = host_cores()
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for fmt, write in (("ply", SC.write_ply), ("splat", SC.write_splat)):
+            path = Path(d) / f"scene.{fmt}"
+            t0 = time.perf_counter()
+            path.write_bytes(write(scene))
+            write_ms = (time.perf_counter() - t0) * 1e3
+            size = path.stat().st_size
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                loaded = SC.load(path, threads=cores)
+                times.append((time.perf_counter() - t0) * 1e3)
+            t0 = time.perf_counter()
+            SC.load(path, threads=1)
+            one_ms = (time.perf_counter() - t0) * 1e3
+            ms = float(np.median(times))
+            rec = {"bytes": size, "write_ms": round(write_ms, 1), "load_ms": round(ms, 2),
+                   "load_ms_1_thread": round(one_ms, 2), "threads": cores,
+                   "GB_per_s": round(size / (ms * 1e-3) / 1e9, 3),
+                   "splats_per_s": round(len(loaded) / (ms * 1e-3), 1)}
+            if fmt == "ply":
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                lr = S.HipRenderer(loaded, timing=False)
+                torch.cuda.synchronize()
+                rec["upload_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+                a, _ = lr.render(view, W, H, (0, 0, 0))
+                b, _ = r.render(view, W, H, (0, 0, 0))
+                p = float(S.psnr(a.pixels, b.pixels))
+                rec["render_psnr_vs_original"] = "inf" if math.isinf(p) else round(p, 2)
+                rec["bytes_differing_vs_original"] = int(np.count_nonzero(a.pixels != b.pixels))
+                lr.close()
+            out[fmt] = rec
+            log(f"scene file {fmt}: {size / 1e6:.1f} MB, load {ms:.1f} ms on {cores} threads, {one_ms:.1f} ms on 1")
+    return out
 
 
 def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:

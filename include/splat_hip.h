@@ -41,6 +41,8 @@ extern "C" {
 #define SP_E_NOMEM (-4)      /* device allocation failed / over the handle's budget */
 #define SP_E_CAPACITY (-5)   /* caller-supplied capacity too small (debug calls) */
 #define SP_E_UNSUPPORTED (-6)
+#define SP_E_FORMAT (-7)     /* malformed or unsupported scene file */
+#define SP_E_IO (-8)         /* scene file cannot be read */
 
 /* Storage precision of the per-splat parameters held on the device. */
 #define SP_STORE_F32 0

@@ -1,4 +1,5 @@
-"""ctypes bindings of the native libraries (C ABI: include/splat_hip.h, include/splat_scene.h).
+"""ctypes bindings of the native libraries (C ABI: include/splat_hip.h, include/splat_scene.h,
+include/splat_io.h).
 
 The rasterizer library is loaded lazily and fails loudly when it is missing:
 there is no Python or CPU fallback for the HIP path.
@@ -20,6 +21,8 @@ SP_E_DEVICE = -3
 SP_E_NOMEM = -4
 SP_E_CAPACITY = -5
 SP_E_UNSUPPORTED = -6
+SP_E_FORMAT = -7
+SP_E_IO = -8
 SP_STORE_F32, SP_STORE_F16, SP_STORE_BF16 = 0, 1, 2
 SP_MAX_DIM = 16384
 TILE = 16
@@ -143,6 +146,21 @@ class SpParamGrads(C.Structure):
     ]
 
 
+class SpIoArrays(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("sh_degree", C.c_uint32),
+        ("rest", C.c_uint32),
+        ("positions", f32p),
+        ("scales", f32p),
+        ("rotations", f32p),
+        ("opacities", f32p),
+        ("colors", f32p),
+        ("sh_rest", f32p),
+        ("cov3d", f32p),
+    ]
+
+
 class SpTuning(C.Structure):
     _fields_ = [
         ("blend_batch", C.c_int32),
@@ -209,6 +227,16 @@ SCENE_SYMBOLS = {
     "sp_to_f16": (None, [f32p, C.c_uint64, u16p]),
     "sp_to_bf16": (None, [f32p, C.c_uint64, u16p]),
     "sp_psnr": (C.c_double, [u8p, u8p, C.c_uint64]),
+    # include/splat_io.h
+    "sp_ply_parse": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sp_splat_parse": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sp_scene_load": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sp_io_finalize": (C.c_int, [C.c_void_p]),
+    "sp_io_arrays_of": (None, [C.c_void_p, C.POINTER(SpIoArrays)]),
+    "sp_io_free": (None, [C.c_void_p]),
+    "sp_io_last_error": (C.c_char_p, []),
+    "sp_ply_write": (C.c_int, [C.POINTER(SpIoArrays), C.c_void_p, C.c_uint64, u64p]),
+    "sp_splat_write": (C.c_int, [C.POINTER(SpIoArrays), C.c_void_p, C.c_uint64, u64p]),
 }
 
 _lock = threading.Lock()

@@ -1,7 +1,8 @@
 """Build the native libraries in-tree.
 
   libsplat_hip.so    HIP rasterizer for gfx950 (hipcc), C ABI in include/splat_hip.h
-  libsplat_scene.so  host scene/camera/workload helpers (g++), include/splat_scene.h
+  libsplat_scene.so  host scene/camera/workload helpers and scene file IO (g++),
+                     include/splat_scene.h, include/splat_io.h
 
 Both are compiled without floating-point contraction: the projection and blend
 reproduce the CPU backend's f32 operation order bit for bit.
@@ -25,7 +26,7 @@ INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 
 HIP_SOURCES = ["scan.hip", "project.hip", "radix.hip", "tiles.hip", "blend.hip", "diff.hip", "api.hip"]
-HOST_SOURCES = ["scene_host.cpp"]
+HOST_SOURCES = ["scene_host.cpp", "scene_io.cpp"]
 ARCH = os.environ.get("SPLAT_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

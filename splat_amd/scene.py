@@ -1,9 +1,9 @@
-"""Scene container and the seeded synthetic workloads.
+"""Scene container, scene files and the seeded synthetic workloads.
 
 `Scene` mirrors splat-tui's structure-of-arrays Scene (src/scene.rs:11-28):
 activated parameters, float32, one row per gaussian. Finalisation,
-covariance, bounds and the random draws run in libsplat_scene.so so that every
-host produces the same bits.
+covariance, bounds, the random draws and the PLY / .splat readers and writers
+run in libsplat_scene.so so that every host produces the same bits.
 """
 from __future__ import annotations
 
@@ -142,6 +142,115 @@ class Scene:
             sh_rest=None if self.sh_rest is None else rnd(self.sh_rest), meta=dict(self.meta),
         )
         return s.finalize()
+
+
+class SceneFileError(ValueError):
+    """A malformed or unsupported scene file (the reference's loader `Err`)."""
+
+
+def _io_check(code: int) -> None:
+    if code != N.SP_OK:
+        msg = N.scene_lib().sp_io_last_error().decode(errors="replace")
+        if code == N.SP_E_IO:
+            raise OSError(msg)
+        raise SceneFileError(msg)
+
+
+class _NativeScene:
+    """Owner of a parsed native scene: freed when the last array viewing it goes."""
+
+    def __init__(self, handle: C.c_void_p):
+        self.handle = handle
+
+    def __del__(self):
+        N.scene_lib().sp_io_free(self.handle)
+
+
+def _take(handle: C.c_void_p) -> Scene:
+    """A Scene whose arrays view a parsed native scene's memory (no copy)."""
+    owner = _NativeScene(handle)
+    a = N.SpIoArrays()
+    N.scene_lib().sp_io_arrays_of(handle, C.byref(a))
+    n, rest = int(a.n), int(a.rest)
+
+    def arr(ptr, shape):
+        count = int(np.prod(shape))
+        if count == 0 or not ptr:
+            return np.zeros(shape, np.float32)
+        mem = (C.c_float * count).from_address(C.addressof(ptr.contents))
+        mem._owner = owner  # keeps the native arrays alive while numpy views them
+        return np.frombuffer(mem, np.float32).reshape(shape)
+
+    return Scene(
+        positions=arr(a.positions, (n, 3)), scales=arr(a.scales, (n, 3)), rotations=arr(a.rotations, (n, 4)),
+        opacities=arr(a.opacities, (n,)), colors=arr(a.colors, (n, 3)), sh_degree=int(a.sh_degree),
+        sh_rest=arr(a.sh_rest, (n, rest, 3)) if rest else None,
+        cov3d=arr(a.cov3d, (n, 6)) if a.cov3d else None,
+    )
+
+
+def _buffer(data) -> np.ndarray:
+    return np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.reshape(-1).view(np.uint8)
+
+
+def parse_ply(data, threads: int = 0) -> Scene:
+    """ply::parse (src/ply.rs:232-379): a PLY file's bytes to an un-finalized Scene."""
+    buf = np.ascontiguousarray(_buffer(data))
+    h = C.c_void_p()
+    _io_check(N.scene_lib().sp_ply_parse(buf.ctypes.data, buf.size, threads, C.byref(h)))
+    return _take(h)
+
+
+def parse_splat(data, threads: int = 0) -> Scene:
+    """splat_format::parse (src/splat_format.rs:18-44): .splat bytes to an un-finalized Scene."""
+    buf = np.ascontiguousarray(_buffer(data))
+    h = C.c_void_p()
+    _io_check(N.scene_lib().sp_splat_parse(buf.ctypes.data, buf.size, threads, C.byref(h)))
+    return _take(h)
+
+
+def load(path, threads: int = 0) -> Scene:
+    """Scene::load (src/scene.rs:97-118): parse by extension, finalize, reject empty scenes."""
+    h = C.c_void_p()
+    _io_check(N.scene_lib().sp_scene_load(str(path).encode(), threads, C.byref(h)))
+    s = _take(h)
+    s.meta = {"workload": "file", "path": str(path)}
+    return s
+
+
+def _io_arrays(scene: Scene) -> tuple[N.SpIoArrays, list]:
+    n = len(scene)
+    rest = rest_count(scene.sh_degree)
+    keep = [np.ascontiguousarray(a, np.float32) for a in
+            (scene.positions, scene.scales, scene.rotations, scene.opacities, scene.colors)]
+    sh = np.ascontiguousarray(scene.sh_rest, np.float32) if rest and scene.sh_rest is not None else None
+    if rest and (sh is None or sh.size != n * rest * 3):
+        raise ValueError("inconsistent scene arrays")
+    keep.append(sh)
+    a = N.SpIoArrays(n, scene.sh_degree, rest, *[N.fptr(k) for k in keep], None)
+    return a, keep
+
+
+def _write(fn, scene: Scene) -> bytes:
+    a, keep = _io_arrays(scene)
+    size = C.c_uint64(0)
+    code = fn(C.byref(a), None, 0, C.byref(size))
+    if code not in (N.SP_OK, N.SP_E_CAPACITY):
+        _io_check(code)
+    out = np.empty(size.value, np.uint8)
+    _io_check(fn(C.byref(a), out.ctypes.data, out.size, C.byref(size)))
+    del keep
+    return out.tobytes()
+
+
+def write_ply(scene: Scene) -> bytes:
+    """ply::write (src/ply.rs:382-431): the standard 3DGS binary little-endian layout."""
+    return _write(N.scene_lib().sp_ply_write, scene)
+
+
+def write_splat(scene: Scene) -> bytes:
+    """splat_format::write (src/splat_format.rs:46-62): 32 bytes per gaussian."""
+    return _write(N.scene_lib().sp_splat_write, scene)
 
 
 def empty_scene() -> Scene:

@@ -17,7 +17,9 @@ def declared(header: str) -> list[str]:
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header,loader", [("splat_hip.h", N.hip), ("splat_scene.h", N.scene_lib)])
+@pytest.mark.parametrize(
+    "header,loader", [("splat_hip.h", N.hip), ("splat_scene.h", N.scene_lib), ("splat_io.h", N.scene_lib)]
+)
 def test_library_exports_every_declared_symbol(header, loader):
     names = declared(header)
     assert len(names) >= 8
@@ -29,6 +31,7 @@ def test_library_exports_every_declared_symbol(header, loader):
 def test_python_bindings_cover_the_header():
     assert set(declared("splat_hip.h")) <= set(N.HIP_SYMBOLS)
     assert set(declared("splat_scene.h")) <= set(N.SCENE_SYMBOLS)
+    assert set(declared("splat_io.h")) <= set(N.SCENE_SYMBOLS)
 
 
 def test_error_surface_without_compute():
