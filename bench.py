@@ -18,6 +18,9 @@ Also reported beside the headline value:
   * scene files: the same scene written as a 3DGS PLY and as .splat, read back
     by the native loader (parse + finalize) and uploaded; load rate per thread
     count and the render of the loaded scene against the original.
+  * orbit_bench: splat-tui's own `bench` table (mean / p50 / p95 per frame
+    with the host readback, PSNR against the first backend's home view) for the
+    hip backend at f32, FP16 and BF16 storage, 1920x1080, 40 frames.
   * differentiation: 500,000 splats, 8 views at 1280x720, L2 loss against
     targets rendered from parameters perturbed by N(0, 0.01): forward and
     backward wall times, and the gradient checked against the FP64 reference
@@ -74,6 +77,7 @@ def main() -> None:
     ap.add_argument("--diff-splats", type=int, default=500_000)
     ap.add_argument("--gradcheck-params", type=int, default=256)
     ap.add_argument("--no-io", action="store_true", help="skip the scene-file leg")
+    ap.add_argument("--no-orbit", action="store_true", help="skip the reference-format orbit table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 5 times (reference kernel for HBM byte counters)")
@@ -232,6 +236,11 @@ def main() -> None:
             result["scene_files"] = bench_scene_io(S, torch, scene, views[0], W, H, r)
         except Exception as e:
             result["scene_files"] = {"error": repr(e)}
+    if not args.no_orbit:
+        try:
+            result["orbit_bench"] = bench_orbit_table(scene, W, H)
+        except Exception as e:
+            result["orbit_bench"] = {"error": repr(e)}
     r.close()
     del r, scene
     if not args.no_diff:
@@ -497,6 +506,30 @@ def bench_scene_io(S, torch, scene, view, W, H, r) -> dict:
             out[fmt] = rec
             log(f"scene file {fmt}: {size / 1e6:.1f} MB, load {ms:.1f} ms on {cores} threads, {one_ms:.1f} ms on 1")
     return out
+
+
+def bench_orbit_table(scene, W, H) -> dict:
+    """splat-tui's `bench <scene> --backend hip,hip-f16,hip-bf16 --size WxH --frames 40`
+    (splat_amd.orbit_bench): per-frame wall times including the readback to host."""
+    from splat_amd import orbit_bench as OB
+
+    backends = ["hip", "hip-f16", "hip-bf16"]
+    res = OB.run(scene, backends, OB.BenchConfig(width=W, height=H, frames=40, warmup=5))
+    table = OB.table(res, backends)
+    for line in table.splitlines():
+        log(line)
+    rows = {}
+    for b, r in zip(backends, res):
+        if isinstance(r, OB.BenchResult):
+            rows[b] = {"mean_ms": round(r.mean_ms, 3), "p50_ms": round(r.p50_ms, 3), "p95_ms": round(r.p95_ms, 3),
+                       "min_ms": round(r.min_ms, 3), "fps": round(1e3 / max(r.mean_ms, 1e-9), 1),
+                       "init_ms": round(r.init_ms, 1), "visible": r.avg.visible, "overlaps": r.avg.overlaps,
+                       "psnr_vs_hip": None if r.psnr_vs_ref is None
+                       else "inf" if math.isinf(r.psnr_vs_ref) else round(r.psnr_vs_ref, 2)}
+        else:
+            rows[b] = {"error": repr(r)}
+    return {"command": f"bench <1M random SH-3> --backend {','.join(backends)} --size {W}x{H} --frames 40",
+            "rows": rows, "table": table.splitlines()}
 
 
 def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:
