@@ -132,6 +132,30 @@ int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
 const char *sp_last_error(void);
 
 /* ---------------------------------------------------------------------------
+ * Terminal output on the device (the viewer's path: src/app.rs:190-230).
+ * --------------------------------------------------------------------------- */
+
+#define SP_TERM_TRUECOLOR 0 /* 6 bytes per cell: foreground r,g,b then background r,g,b */
+#define SP_TERM_ANSI256 1   /* 2 bytes per cell: foreground, background xterm-256 index */
+
+/* term::downsample (src/term.rs:75-101) on device frames: `src` (width*height*3)
+ * box-filtered by an integer factor (0 acts as 1) into `dst`
+ * ((width/factor)*(height/factor)*3), each channel the truncated mean of its
+ * block. Returns when `dst` is complete. */
+int sp_downsample_device(sp_handle *h, const uint8_t *src, uint32_t width, uint32_t height, uint32_t factor,
+                         uint8_t *dst);
+
+/* The viewer's frame as terminal cells: render the view at
+ * (cols*factor) x (2*rows*factor), downsample by `factor`, and write the
+ * half-block cells of the cols x (2*rows) result (HalfBlockImage,
+ * src/term.rs:103-128; rgb_to_ansi256, src/term.rs:48-72) to the host,
+ * row-major: cell (cx, cy) has display pixel (cx, 2cy) as its foreground and
+ * (cx, 2cy+1) as its background, in the `mode` layout above. Only the cells
+ * cross PCIe. A rendered side above SP_MAX_DIM returns SP_E_PIXEL_LIMIT. */
+int sp_render_cells(sp_handle *h, const sp_view *view, uint32_t cols, uint32_t rows, uint32_t factor,
+                    const float background[3], int32_t mode, uint8_t *cells_out, sp_stats *stats);
+
+/* ---------------------------------------------------------------------------
  * Inspection entry points (the oracle comparison surface). They have no direct
  * counterpart in the reference's public API; they expose the intermediate
  * arrays of CpuTiled::render (src/render/cpu.rs:131-200) in canonical form.

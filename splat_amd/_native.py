@@ -24,6 +24,7 @@ SP_E_UNSUPPORTED = -6
 SP_E_FORMAT = -7
 SP_E_IO = -8
 SP_STORE_F32, SP_STORE_F16, SP_STORE_BF16 = 0, 1, 2
+SP_TERM_TRUECOLOR, SP_TERM_ANSI256 = 0, 1
 SP_MAX_DIM = 16384
 TILE = 16
 
@@ -182,6 +183,11 @@ HIP_SYMBOLS = {
         [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, f32p, C.c_void_p, C.POINTER(SpStats)],
     ),
     "sp_last_error": (C.c_char_p, []),
+    "sp_downsample_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "sp_render_cells": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_int32, u8p, C.POINTER(SpStats)],
+    ),
     "sp_debug_project": (C.c_int, [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, u8p, f32p, u16p]),
     "sp_debug_pairs": (
         C.c_int,

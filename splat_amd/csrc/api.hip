@@ -928,6 +928,47 @@ int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height
     return SP_OK;
 }
 
+int sp_downsample_device(sp_handle *h, const uint8_t *src, uint32_t width, uint32_t height, uint32_t factor,
+                         uint8_t *dst) {
+    if (!h) return fail(SP_E_INVALID, "sp_downsample_device: NULL handle");
+    factor = std::max(factor, 1u);
+    const uint32_t dw = width / factor, dh = height / factor;
+    if ((uint64_t)dw * dh == 0) return SP_OK;
+    if (!src || !dst) return fail(SP_E_INVALID, "sp_downsample_device: NULL frame");
+    int rc = setup(h);
+    if (rc) return rc;
+    HIP_TRY(sp::launch_downsample(h->stream, src, width, factor, dw, dh, dst));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SP_OK;
+}
+
+int sp_render_cells(sp_handle *h, const sp_view *view, uint32_t cols, uint32_t rows, uint32_t factor,
+                    const float background[3], int32_t mode, uint8_t *cells_out, sp_stats *stats) {
+    if (!h || !view || !background) return fail(SP_E_INVALID, "sp_render_cells: NULL argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (mode != SP_TERM_TRUECOLOR && mode != SP_TERM_ANSI256) return fail(SP_E_INVALID, "sp_render_cells: bad mode");
+    factor = std::max(factor, 1u);
+    const uint64_t w = (uint64_t)cols * factor, hh = 2ull * rows * factor;
+    if (w > SP_MAX_DIM || hh > SP_MAX_DIM) return fail(SP_E_PIXEL_LIMIT, "sp_render_cells: pixel limit");
+    if (w == 0 || hh == 0) return SP_OK;
+    if (!cells_out) return fail(SP_E_INVALID, "sp_render_cells: NULL output");
+    int rc = setup(h);
+    if (rc) return rc;
+    // The frame and its cells share the handle's RGB buffer: cells after the frame.
+    const uint64_t bytes = 3 * w * hh;
+    const uint64_t cell_bytes = (mode == SP_TERM_ANSI256 ? 2ull : 6ull) * cols * rows;
+    rc = ensure_rgb(h, bytes + cell_bytes);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    rc = render_one(h, view, (uint32_t)w, (uint32_t)hh, background, h->d_rgb, stats);
+    if (rc) return rc;
+    HIP_TRY(sp::launch_cells(h->stream, h->d_rgb, (uint32_t)w, factor, cols, rows, mode, h->d_rgb + bytes));
+    HIP_TRY(hipMemcpyAsync(cells_out, h->d_rgb + bytes, cell_bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (stats) stats->total_ms = now_ms() - t0;
+    return SP_OK;
+}
+
 int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, uint8_t *valid,
                      float *proj, uint16_t *bbox) {
     if (!h || !view || !valid || !proj || !bbox) return fail(SP_E_INVALID, "sp_debug_project: NULL argument");

@@ -13,6 +13,9 @@ constexpr float ALPHA_MIN = 1.0f / 255.0f;
 constexpr float ALPHA_MAX = 0.99f;
 constexpr float T_MIN = 1e-4f;
 constexpr uint32_t TILE = 16;
+// Terminal cell layouts (SP_TERM_TRUECOLOR / SP_TERM_ANSI256 of the C ABI).
+constexpr int32_t TERM_TRUECOLOR = 0;
+constexpr int32_t TERM_ANSI256 = 1;
 // Superblock: SB x SB tiles, the bin of the first emission phase (tiles.hip).
 constexpr uint32_t SB = 4;
 constexpr uint32_t BIN_SCAN_ROW = 2048;  // tiles per row of the binned tile-count scan

@@ -135,4 +135,13 @@ hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParam
 // p[0..n) = value for float arrays.
 hipError_t launch_fill_f32(hipStream_t s, float *p, uint64_t n, float value);
 
+// ---- term.hip: terminal output ----
+// dst (dw x dh RGB8) = src (w wide) box-filtered by factor >= 1 (term::downsample).
+hipError_t launch_downsample(hipStream_t s, const uint8_t *src, uint32_t w, uint32_t factor, uint32_t dw,
+                             uint32_t dh, uint8_t *dst);
+// Half-block cells (cols x rows) of src (w wide) downsampled by factor >= 1:
+// 6 bytes per cell (TERM_TRUECOLOR) or 2 (TERM_ANSI256).
+hipError_t launch_cells(hipStream_t s, const uint8_t *src, uint32_t w, uint32_t factor, uint32_t cols, uint32_t rows,
+                        int32_t mode, uint8_t *cells);
+
 }  // namespace sp

@@ -122,6 +122,23 @@ class HipRenderer:
         stats = RenderStats.from_c(st)
         return (frame, stats) if made else stats
 
+    def render_cells(self, view: N.SpView, cols: int, rows: int, factor: int, background, mode: str = "truecolor"):
+        """The viewer's frame as half-block terminal cells (sp_render_cells): rendered at
+        (cols*factor) x (2*rows*factor), box-filtered by `factor`, each cell the pair
+        (pixel (cx, 2cy), pixel (cx, 2cy+1)). Returns (cells, stats): cells is
+        rows x cols x 6 RGB bytes (truecolor) or rows x cols x 2 xterm-256 indices (ansi256)."""
+        m = {"truecolor": (N.SP_TERM_TRUECOLOR, 6), "ansi256": (N.SP_TERM_ANSI256, 2)}[mode]
+        cells = np.zeros((rows, cols, m[1]), np.uint8)
+        st = N.SpStats()
+        N.check(self._lib.sp_render_cells(self._h, C.byref(view), cols, rows, factor, N.fptr(_bg(background)), m[0],
+                                          cells.ctypes.data_as(N.u8p), C.byref(st)))
+        return cells, RenderStats.from_c(st)
+
+    def downsample_device(self, src_ptr: int, width: int, height: int, factor: int, dst_ptr: int) -> None:
+        """term::downsample of a device RGB8 frame into device memory (sp_downsample_device)."""
+        N.check(self._lib.sp_downsample_device(self._h, C.c_void_p(src_ptr), width, height, factor,
+                                               C.c_void_p(dst_ptr)))
+
     def render_device(self, view: N.SpView, width: int, height: int, background, out_ptr: int) -> RenderStats:
         """Render into device memory at `out_ptr` (width*height*3 bytes); no host copy."""
         st = N.SpStats()
