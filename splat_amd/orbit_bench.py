@@ -14,7 +14,7 @@ Backends here are the HIP rasterizer at its three storage precisions: `hip`
 `hip` first, the PSNR column shows what each 16-bit storage costs.
 
   python -m splat_amd.orbit_bench SCENE [--backend hip,hip-f16] [--size 320x180]
-                                        [--frames 60] [--warmup 5] [--up -y]
+                                        [--frames 60] [--warmup 5] [--up -y] [--no-stage-times]
 
 SCENE is a .ply / .splat file, `random:N[:DEGREE]` (the seeded random
 workload) or `lattice` (the 3072-splat lattice).
@@ -52,6 +52,9 @@ class BenchConfig:
     frames: int = 60
     warmup: int = 5
     up: UpAxis = UpAxis.NEG_Y
+    # Stage times (project / sort / raster columns) from HIP events between the
+    # stages; without them each frame is one replay of a captured graph.
+    stage_times: bool = True
 
 
 @dataclass
@@ -75,15 +78,17 @@ def percentile(sorted_ms: list[float], p: float) -> float:
     return sorted_ms[int(math.floor((len(sorted_ms) - 1) * p + 0.5))]
 
 
-def _create(name: str, scene):
+def _create(name: str, scene, stage_times: bool = True):
     if name not in BACKENDS:
         raise ValueError(f"unknown backend '{name}' (available: {', '.join(BACKENDS)})")
-    # Frames only: each render is one replay of the handle's captured graph.
-    return HipRenderer(scene, storage=BACKENDS[name], timing=False)
+    return HipRenderer(scene, storage=BACKENDS[name], timing=stage_times)
 
 
-def run(scene, backends: list[str], cfg: BenchConfig, create=_create) -> list:
+def run(scene, backends: list[str], cfg: BenchConfig, create=None) -> list:
     """bench::run (src/bench.rs:38-118): a BenchResult, or the exception, per backend."""
+    if create is None:
+        def create(name, sc):
+            return _create(name, sc, cfg.stage_times)
     center, radius = scene.bounds()
     reference = None
     out = []
@@ -182,6 +187,8 @@ def main(argv=None) -> int:
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--up", default="-y")
+    ap.add_argument("--no-stage-times", action="store_true",
+                    help="frames only: each render replays a captured graph (stage columns read 0)")
     a = ap.parse_args(argv)
     try:
         w, h = parse_size(a.size)
@@ -194,7 +201,7 @@ def main(argv=None) -> int:
     print(f"{a.scene}: {len(scene)} gaussians (SH degree {scene.sh_degree}), loaded in "
           f"{(time.perf_counter() - t) * 1e3:.0f} ms; {w}x{h} px, {a.frames} frames, {os.cpu_count()} threads")
     backends = [b for b in a.backend.split(",") if b]
-    res = run(scene, backends, BenchConfig(w, h, a.frames, a.warmup, up))
+    res = run(scene, backends, BenchConfig(w, h, a.frames, a.warmup, up, not a.no_stage_times))
     print(table(res, backends))
     return 0 if all(isinstance(r, BenchResult) for r in res) else 1
 
