@@ -108,7 +108,7 @@ def test_orbit_bench_on_the_gpu(capsys):
     assert res[0].avg.overlaps > 0 and res[0].mean_ms > 0
     assert res[0].avg.raster_ms > 0 and res[0].avg.project_ms > 0  # stage times from HIP events
     fast = B.run(scene, ["hip"], B.BenchConfig(width=160, height=90, frames=3, warmup=0, stage_times=False))
-    assert fast[0].avg.raster_ms == 0 and fast[0].avg.overlaps == res[0].avg.overlaps or fast[0].frames == 3
+    assert isinstance(fast[0], B.BenchResult) and fast[0].avg.raster_ms == 0 and fast[0].avg.overlaps > 0
     assert B.main(["random:5000:1", "--size", "64x48", "--frames", "2", "--warmup", "0"]) == 0
     out = capsys.readouterr().out
     assert "random:5000:1: 5000 gaussians (SH degree 1)" in out and "64x48 px, 2 frames" in out
