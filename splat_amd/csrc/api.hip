@@ -145,7 +145,7 @@ struct sp_handle {
     uint32_t *d_bcnt_all = nullptr;  // view_cap x block counts of the unsplit emission
     uint32_t *d_bsb = nullptr;    // view_cap x block counts of superblock pairs (binned first phase)
     ViewParams *d_views = nullptr, *h_views = nullptr;
-    uint32_t *d_rowtot = nullptr;  // radix digit-row totals: view_cap x 256 digits
+    uint32_t *d_rowtot = nullptr;  // radix digit-row totals: view_cap x RADIX_BINS_MAX digits
     uint32_t *d_table = nullptr;  // radix per-block digit counts
     uint64_t table_words = 0;
     // Pairs of all views.
@@ -313,7 +313,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     dfree(h, h->d_bcnt_all, old * blocks * 4);
     dfree(h, h->d_bsb, old * blocks * 4);
     dfree(h, h->d_views, old * sizeof(ViewParams));
-    dfree(h, h->d_rowtot, old * sp::RADIX_BINS * 4);
+    dfree(h, h->d_rowtot, old * sp::RADIX_BINS_MAX * 4);
     if (h->h_views) (void)hipHostFree(h->h_views);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     h->h_views = nullptr;
@@ -332,7 +332,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt_all, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bsb, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_views, v * sizeof(ViewParams));
-    if (!rc) rc = dalloc(h, (void **)&h->d_rowtot, v * sp::RADIX_BINS * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_rowtot, v * sp::RADIX_BINS_MAX * 4);
     if (rc) return rc;
     if (hipHostMalloc((void **)&h->h_views, v * sizeof(ViewParams), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&h->h_counters, (sp::CNT_SLOTS + 2 * v) * 4, hipHostMallocDefault) != hipSuccess)

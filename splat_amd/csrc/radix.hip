@@ -1,13 +1,14 @@
 // (c) Stable LSD radix sort of u32 keys carrying u32 values, over independent
 // segments (one per view for the depth sort, one for the tile sort).
 //
-// Per 8-bit digit:
+// Per digit (8 bits, 256 bins; or 9 bits, 512 bins, in a pass that saves a
+// pass, e.g. a 9-bit sort in one pass instead of 8 + 1):
 //   * upsweep: each block counts the digits of a tile (4096 keys, or 1024 for
 //     sorts small enough that 4096-key tiles would leave CUs idle) into a
 //     (segment, digit, column) table, a column being 1-8 consecutive tiles,
 //   * digit-row scan: one workgroup per (segment, digit) row turns the counts
 //     into in-row positions and the row's total,
-//   * downsweep: each block scans its segment's 256 row totals into digit
+//   * downsweep: each block scans its segment's row totals into digit
 //     bases (so no histogram pass is needed), re-reads its tile, ranks it with a wave64 ballot
 //     multisplit (8 ballots give the mask of lanes sharing the digit; the rank
 //     is the popcount below the lane plus a per-wave LDS counter, so ranks
@@ -36,17 +37,18 @@ namespace {
 constexpr int NWAVES = RADIX_THREADS / 64;
 constexpr uint32_t RADIX_GRID = 4096;  // bound on the upsweep / downsweep grid
 
-// table[(seg * 256 + d) * bps + c] = count of digit d in table column c of
+// table[(seg * BINS + d) * bps + c] = count of digit d in table column c of
 // `seg` (radix_group consecutive tiles of 64 * NWAVES * ITEMS keys). gps workgroups per segment
 // stride over the segment's live columns, so the grid follows the capacity
 // only up to a bound and idle work follows the live count.
-template <int ITEMS>
+template <int ITEMS, int BINS>
 __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ seg_counts,
                                                            uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
                                                            uint32_t gps, int shift, int width,
                                                            uint32_t *__restrict__ table) {
-    __shared__ uint32_t s_h[NWAVES][RADIX_BINS];
+    __shared__ uint32_t s_h[NWAVES][BINS];
+    constexpr int DPT = BINS / RADIX_THREADS;  // digits per thread
     const uint32_t dmask = (1u << width) - 1u;
     const uint32_t seg = blockIdx.x / gps;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
     const uint32_t tiles = radix_tiles(n, TILE), group = radix_group(tiles), cols = (tiles + group - 1) / group;
     const uint32_t *ks = kin + seg * seg_stride;
     for (uint32_t col = blockIdx.x % gps; col < cols; col += gps) {
-        for (uint32_t k = threadIdx.x; k < NWAVES * RADIX_BINS; k += RADIX_THREADS) (&s_h[0][0])[k] = 0;
+        for (uint32_t k = threadIdx.x; k < NWAVES * BINS; k += RADIX_THREADS) (&s_h[0][0])[k] = 0;
         __syncthreads();
         const uint32_t t_end = min(tiles, (col + 1) * group);
         for (uint32_t t = col * group; t < t_end; ++t) {
@@ -68,12 +70,12 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
                 const uint32_t i = w * WAVE_KEYS + k * 64 + lane;
-                dg[k] = i < rem ? (kt[i] >> shift) & dmask : RADIX_BINS;  // RADIX_BINS: no key
+                dg[k] = i < rem ? (kt[i] >> shift) & dmask : BINS;  // BINS: no key
             }
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
                 const uint32_t d = dg[k];
-                const bool valid = d < RADIX_BINS;
+                const bool valid = d < BINS;
                 // Sorted or clustered keys often give a whole wave one digit: one
                 // add then replaces 64 colliding LDS atomics (s_h[w] is this wave's own).
                 const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
@@ -86,31 +88,35 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
             }
         }
         __syncthreads();
-        const uint32_t d = threadIdx.x;
-        uint32_t c = 0;
 #pragma unroll
-        for (int ww = 0; ww < NWAVES; ++ww) c += s_h[ww][d];
-        table[((uint64_t)seg * RADIX_BINS + d) * bps + col] = c;
+        for (int j = 0; j < DPT; ++j) {
+            const uint32_t d = threadIdx.x * DPT + j;
+            uint32_t c = 0;
+#pragma unroll
+            for (int ww = 0; ww < NWAVES; ++ww) c += s_h[ww][d];
+            table[((uint64_t)seg * BINS + d) * bps + col] = c;
+        }
         __syncthreads();
     }
 }
 
 // One tile of RADIX_THREADS * ITEMS keys: rank, exchange through LDS into
 // digit order, write.
-// `run` is thread d's next segment position for digit d (the column base plus
-// the keys of digit d in the column's earlier tiles); it advances past this
-// tile's keys of digit d.
+// run[j] is the next segment position of digit threadIdx.x * DPT + j (the
+// column base plus the keys of that digit in the column's earlier tiles); it
+// advances past this tile's keys of the digit.
 // The exchange buffer has one pad entry per 32, so the runs of one digit
 // (about 16 keys long when the keys spread over all digits, 128 bytes apart)
 // start in different LDS banks; four large-tile workgroups still fit a CU.
 __device__ __forceinline__ uint32_t ex_slot(uint32_t p) { return p + (p >> 5); }
 
-template <int ITEMS>
+template <int ITEMS, int BINS>
 __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, const uint32_t *__restrict__ vs,
                                                uint32_t *__restrict__ ko, uint32_t *__restrict__ vo, uint64_t n,
-                                               uint32_t t, int shift, int width, uint32_t &run,
-                                               uint32_t (*s_wh)[RADIX_BINS], uint32_t *s_lstart, uint32_t *s_gbase,
+                                               uint32_t t, int shift, int width, uint32_t *run,
+                                               uint32_t (*s_wh)[BINS], uint32_t *s_lstart, uint32_t *s_gbase,
                                                uint2 *s_ex) {
+    constexpr int DPT = BINS / RADIX_THREADS;
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS, WAVE_KEYS = 64 * ITEMS;
     const uint64_t base = (uint64_t)t * TILE;
     const uint32_t rem = (uint32_t)min((uint64_t)TILE, n - base);
@@ -118,7 +124,7 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     const uint32_t *vt = vs + base;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t dmask = (1u << width) - 1u;
-    for (uint32_t k = threadIdx.x; k < NWAVES * RADIX_BINS; k += RADIX_THREADS) (&s_wh[0][0])[k] = 0;
+    for (uint32_t k = threadIdx.x; k < NWAVES * BINS; k += RADIX_THREADS) (&s_wh[0][0])[k] = 0;
     __syncthreads();
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -157,19 +163,33 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
         if (valid && below == 0) s_wh[w][d] = before + cnt;
     }
     __syncthreads();
-    const uint32_t d = threadIdx.x;
-    uint32_t bc = 0;
+    // Per digit: the waves' counts become per-wave offsets; bc[j] = the tile's
+    // keys of digit threadIdx.x * DPT + j. Consecutive digits per thread, so one
+    // block scan of the threads' sums gives every digit's tile-local start.
+    uint32_t bc[DPT], tsum = 0;
 #pragma unroll
-    for (int ww = 0; ww < NWAVES; ++ww) {
-        const uint32_t c = s_wh[ww][d];
-        s_wh[ww][d] = bc;
-        bc += c;
+    for (int j = 0; j < DPT; ++j) {
+        const uint32_t d = threadIdx.x * DPT + j;
+        uint32_t b = 0;
+#pragma unroll
+        for (int ww = 0; ww < NWAVES; ++ww) {
+            const uint32_t c = s_wh[ww][d];
+            s_wh[ww][d] = b;
+            b += c;
+        }
+        bc[j] = b;
+        tsum += b;
     }
     uint32_t tot;
-    const uint32_t lstart = block_exclusive_scan<RADIX_THREADS>(bc, &tot);
-    s_lstart[d] = lstart;
-    s_gbase[d] = run - lstart;
-    run += bc;
+    uint32_t lstart = block_exclusive_scan<RADIX_THREADS>(tsum, &tot);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const uint32_t d = threadIdx.x * DPT + j;
+        s_lstart[d] = lstart;
+        s_gbase[d] = run[j] - lstart;
+        run[j] += bc[j];
+        lstart += bc[j];
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
@@ -189,7 +209,7 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     }
 }
 
-template <int ITEMS>
+template <int ITEMS, int BINS>
 __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__restrict__ kin,
                                                              const uint32_t *__restrict__ vin,
                                                              uint32_t *__restrict__ kout,
@@ -199,28 +219,41 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
                                                              uint32_t gps, int shift, int width,
                                                              const uint32_t *__restrict__ table,
                                                              const uint32_t *__restrict__ row_total) {
-    __shared__ uint32_t s_wh[NWAVES][RADIX_BINS];  // per-wave digit counters, then per-wave offsets
-    __shared__ uint32_t s_lstart[RADIX_BINS];      // block-local start of each digit
-    __shared__ uint32_t s_gbase[RADIX_BINS];       // segment position minus local position
+    __shared__ uint32_t s_wh[NWAVES][BINS];  // per-wave digit counters, then per-wave offsets
+    __shared__ uint32_t s_lstart[BINS];      // block-local start of each digit
+    __shared__ uint32_t s_gbase[BINS];       // segment position minus local position
+    constexpr int DPT = BINS / RADIX_THREADS;
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
     __shared__ uint2 s_ex[TILE + TILE / 32];       // key/value exchange in digit order (padded)
     const uint32_t seg = blockIdx.x / gps;
     const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
     const uint32_t tiles = radix_tiles(n, TILE), group = radix_group(tiles), cols = (tiles + group - 1) / group;
     if (blockIdx.x % gps >= cols) return;  // block-uniform
-    // Segment position of digit threadIdx.x's first key: the row totals of the lower digits.
+    // Segment position of each of this thread's digits' first key: the row
+    // totals of the lower digits.
+    uint32_t rt[DPT], rsum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        rt[j] = row_total[(uint64_t)seg * BINS + threadIdx.x * DPT + j];
+        rsum += rt[j];
+    }
     uint32_t seg_total;
-    const uint32_t row_base =
-        block_exclusive_scan<RADIX_THREADS>(row_total[(uint64_t)seg * RADIX_BINS + threadIdx.x], &seg_total);
+    uint32_t row_base[DPT];
+    row_base[0] = block_exclusive_scan<RADIX_THREADS>(rsum, &seg_total);
+#pragma unroll
+    for (int j = 1; j < DPT; ++j) row_base[j] = row_base[j - 1] + rt[j - 1];
     const uint32_t *ks = kin + seg * seg_stride;
     const uint32_t *vs = vin + seg * seg_stride;
     uint32_t *ko = kout + seg * seg_stride;
     uint32_t *vo = vout + seg * seg_stride;
     for (uint32_t col = blockIdx.x % gps; col < cols; col += gps) {
-        uint32_t run = row_base + table[((uint64_t)seg * RADIX_BINS + threadIdx.x) * bps + col];
+        uint32_t run[DPT];
+#pragma unroll
+        for (int j = 0; j < DPT; ++j)
+            run[j] = row_base[j] + table[((uint64_t)seg * BINS + threadIdx.x * DPT + j) * bps + col];
         const uint32_t t_end = min(tiles, (col + 1) * group);
         for (uint32_t t = col * group; t < t_end; ++t) {
-            downsweep_tile<ITEMS>(ks, vs, ko, vo, n, t, shift, width, run, s_wh, s_lstart, s_gbase, s_ex);
+            downsweep_tile<ITEMS, BINS>(ks, vs, ko, vo, n, t, shift, width, run, s_wh, s_lstart, s_gbase, s_ex);
             __syncthreads();  // LDS is reused by the next tile
         }
     }
@@ -230,12 +263,27 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
 
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap) {
     const uint32_t tk = radix_tile_keys(nseg, n_cap);
-    return (uint64_t)nseg * RADIX_BINS * ((n_cap + tk - 1) / tk);
+    return (uint64_t)nseg * RADIX_BINS_MAX * ((n_cap + tk - 1) / tk);
 }
 
 uint64_t radix_table_bound(uint32_t max_seg) {
     const uint64_t small_tile = (uint64_t)RADIX_THREADS * RADIX_ITEMS_SMALL;
-    return (uint64_t)RADIX_BINS * (RADIX_SMALL_MAX / small_tile + max_seg);
+    return (uint64_t)RADIX_BINS_MAX * (RADIX_SMALL_MAX / small_tile + max_seg);
+}
+
+int radix_plan(int bits, int *widths) {
+    // 8-bit digits, except that passes of 9 bits are used when they need fewer
+    // passes (9 -> 9, 17 -> 9+8, 18 -> 9+9, 25 -> 9+8+8, 26 -> 9+9+8, 27 -> 9+9+9).
+    const int p8 = (bits + 7) / 8, p9 = (bits + 8) / 9;
+    const int passes = std::min(p8, p9);
+    int nine = p9 < p8 ? bits - 8 * passes : 0;
+    int left = bits;
+    for (int p = 0; p < passes; ++p) {
+        widths[p] = std::min(left, nine > 0 ? 9 : 8);
+        if (nine > 0) --nine;
+        left -= widths[p];
+    }
+    return passes;
 }
 
 namespace {
@@ -244,24 +292,38 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], con
                         uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
                         uint32_t *table, int *out_index) {
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
-    const int passes = (bits + 7) / 8;
+    int widths[8];
+    const int passes = radix_plan(bits, widths);
     int cur = 0;
     const uint32_t bps = (uint32_t)((n_cap + TILE - 1) / TILE);
     // Workgroups per segment: one per column up to a grid bound; they stride
     // over the live columns, so a capacity far above the live count costs little.
     const uint32_t gps = std::max<uint32_t>(1, std::min<uint32_t>(bps, std::max<uint32_t>(1, RADIX_GRID / nseg)));
+    int shift = 0;
     for (int p = 0; p < passes; ++p) {
-        const int width = std::min(8, bits - 8 * p);  // digit bits this pass carries
-        hipLaunchKernelGGL(k_upsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], seg_counts,
-                           seg_stride, n_cap, bps, gps, 8 * p, width, table);
+        const int width = widths[p];  // digit bits this pass carries
+        const uint32_t bins = width > 8 ? 512u : 256u;
+        if (bins == 512)
+            hipLaunchKernelGGL((k_upsweep<ITEMS, 512>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                               seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
+        else
+            hipLaunchKernelGGL((k_upsweep<ITEMS, 256>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                               seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = launch_scan_digit_rows(s, table, bps, nseg, seg_counts, n_cap, TILE, row_total);
+        if (e == hipSuccess)
+            e = launch_scan_digit_rows(s, table, bps, nseg, bins, seg_counts, n_cap, TILE, row_total);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_downsweep<ITEMS>, dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur], vals[cur],
-                           keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps, 8 * p, width,
-                           table, row_total);
+        if (bins == 512)
+            hipLaunchKernelGGL((k_downsweep<ITEMS, 512>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                               vals[cur], keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps,
+                               shift, width, table, row_total);
+        else
+            hipLaunchKernelGGL((k_downsweep<ITEMS, 256>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                               vals[cur], keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps,
+                               shift, width, table, row_total);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+        shift += width;
         cur = 1 - cur;
     }
     *out_index = cur;

@@ -72,13 +72,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, 
 // counts, of which the first radix_columns(min(seg_counts[g], n_cap),
 // tile_keys) are live.
 // Each live prefix becomes its in-row exclusive scan and row_total[r] its sum
-// (the sort's downsweep turns the 256 row totals of a segment into digit
+// (the sort's downsweep turns the row totals of a segment into digit
 // bases itself, so no separate histogram pass is needed).
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
+                                                                  uint32_t bins,
                                                                   const uint32_t *__restrict__ seg_counts,
                                                                   uint64_t n_cap, uint32_t tile_keys,
                                                                   uint32_t *__restrict__ row_total) {
-    const uint32_t g = blockIdx.x / RADIX_BINS;
+    const uint32_t g = blockIdx.x / bins;
     const uint64_t n = min((uint64_t)seg_counts[g], n_cap);
     const uint64_t live = min(row_len, (uint64_t)radix_columns(n, tile_keys));
     uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
@@ -119,11 +120,11 @@ hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *d
     return hipGetLastError();
 }
 
-hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
+hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg, uint32_t bins,
                                   const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
                                   uint32_t *row_total) {
     if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * RADIX_BINS), dim3(SCAN_THREADS), 0, s, data, row_len,
+    hipLaunchKernelGGL(k_scan_digit_rows, dim3(nseg * bins), dim3(SCAN_THREADS), 0, s, data, row_len, bins,
                        seg_counts, n_cap, tile_keys, row_total);
     return hipGetLastError();
 }

@@ -27,7 +27,7 @@ constexpr int PROJECT_THREADS = 256;  // one splat per thread
 constexpr int PROJECT_VCHUNK = 8;     // views projected per pass over a splat held in registers
 constexpr int EMIT_THREADS = 256;     // one depth-ordered splat per thread
 constexpr int BLEND_THREADS = 256;    // one pixel of a 16x16 tile per thread
-constexpr int RADIX_THREADS = 256;    // one digit per thread in the lookback
+constexpr int RADIX_THREADS = 256;    // one or two digits per thread
 // Keys per thread of a radix tile: 16 (4096-key tiles) for large sorts, 4
 // (1024-key tiles) for sorts whose segments hold at most RADIX_SMALL_MAX keys
 // together, whose 4096-key tiles would leave most CUs idle.
@@ -38,6 +38,9 @@ __host__ __device__ constexpr uint32_t radix_tile_keys(uint32_t nseg, uint64_t n
     return (uint32_t)(RADIX_THREADS * ((uint64_t)nseg * n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
 }
 constexpr int RADIX_BINS = 256;       // 8-bit digits
+// A pass may carry a 9-bit digit (512 bins) when that saves a pass (a 9-bit
+// sort in one pass instead of 8 + 1); tables are sized for it.
+constexpr int RADIX_BINS_MAX = 512;
 constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
 // Device counter slots (uint32 each) in the per-handle counter block.

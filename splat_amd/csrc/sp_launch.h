@@ -17,10 +17,10 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 // The same for up to 3 arrays of `rows` rows each, in one launch.
 hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
                                   uint64_t row_len, uint32_t rows);
-// Radix tables: rows (segment g, digit d); only the live prefix of each row
+// Radix tables: rows (segment g, digit d < bins); only the live prefix of each row
 // (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries) is scanned,
 // in place and from 0; row_total[g * 256 + d] receives the row's sum.
-hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg,
+hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg, uint32_t bins,
                                   const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
                                   uint32_t *row_total);
 // p[0..n) = value (a kernel, so it may be captured in a graph).
@@ -44,12 +44,15 @@ hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *v
 // holding at most RADIX_SMALL_MAX keys together) of up to max_seg segments.
 uint64_t radix_table_words(uint32_t nseg, uint64_t n_cap);
 uint64_t radix_table_bound(uint32_t max_seg);
+// Pass plan of a `bits`-bit sort: digit widths (8, or 9 where that saves a
+// pass) into widths[], returns the number of passes.
+int radix_plan(int bits, int *widths);
 // Sorts u32 keys with u32 values by bits [0, bits) (stable; higher key bits
-// travel with the key without ordering it), 8-bit digits, over
+// travel with the key without ordering it), 8- or 9-bit digits, over
 // `nseg` independent segments: segment g holds min(seg_counts[g], n_cap) keys
 // at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
 // between keys[0]/keys[1]; *out_index names the buffer with the result.
-// row_total holds nseg * 256 u32, table radix_table_words(nseg, n_cap) u32.
+// row_total holds nseg * RADIX_BINS_MAX u32, table radix_table_words(nseg, n_cap) u32.
 hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
                              uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
                              uint32_t *table, int *out_index);

@@ -260,3 +260,22 @@ def test_scene_files_render_like_cpu(oracle, tmp_path):
             assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
             assert np.array_equal(img.pixels, ref), f"{name} view {vi}"
         r.close()
+
+
+@pytest.mark.parametrize("w,h", [(1280, 1024), (1920, 1080), (5000, 4000)])
+def test_nine_bit_radix_passes(w, h, oracle):
+    """Frame sizes whose sorts plan a 9-bit digit: 320 and 510 superblocks (one
+    9-bit pass instead of 8 + 1) and 78250 tiles (a 17-bit tile sort as 9 + 8).
+    Pairs, ranges and the image stay bit-exact against the cpu restatement."""
+    s = scene("random20k")
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    v = _views(s, 1)[0]
+    case(f"nine-bit passes {w}x{h}")
+    gk, gv, gt = r.debug_pairs(v, w, h)
+    ok, ov, ot = cpu.pairs(v, w, h)
+    assert np.array_equal(gt, ot) and np.array_equal(gk, ok) and np.array_equal(gv, ov)
+    img, _ = r.render(v, w, h, BG)
+    ref, _ = cpu.render(v, w, h, BG)
+    assert np.array_equal(img.pixels, ref)
+    r.close()
