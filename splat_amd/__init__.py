@@ -6,7 +6,8 @@ over it plus the scene/camera helpers the drivers need.
 """
 from .camera import OrbitCamera, UpAxis, orbit_views, random_views, view_at, view_of
 from .renderer import Backend, Frame, HipRenderer, RenderStats, create, psnr
-from .scene import Scene, empty_scene, lattice, random_scene, scene_of, test_scene
+from .scene import (Scene, SceneFileError, empty_scene, lattice, load, parse_ply, parse_splat, random_scene, scene_of,
+                    test_scene, write_ply, write_splat)
 
 __all__ = [
     "Backend",
@@ -15,11 +16,15 @@ __all__ = [
     "OrbitCamera",
     "RenderStats",
     "Scene",
+    "SceneFileError",
     "UpAxis",
     "create",
     "empty_scene",
     "lattice",
+    "load",
     "orbit_views",
+    "parse_ply",
+    "parse_splat",
     "psnr",
     "random_scene",
     "random_views",
@@ -27,4 +32,6 @@ __all__ = [
     "test_scene",
     "view_at",
     "view_of",
+    "write_ply",
+    "write_splat",
 ]
