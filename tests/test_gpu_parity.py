@@ -279,3 +279,22 @@ def test_nine_bit_radix_passes(w, h, oracle):
     ref, _ = cpu.render(v, w, h, BG)
     assert np.array_equal(img.pixels, ref)
     r.close()
+
+
+@pytest.mark.parametrize("w,h", [(16384, 16), (16, 16384), (3840, 2160)])
+def test_largest_sides_and_4k(w, h, oracle):
+    """The largest side the reference allows (16384, src/render/mod.rs:35-41) at
+    extreme aspect ratios, and a 4K frame: pairs, ranges and image bit-exact."""
+    s = scene("random20k")
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    v = _views(s, 1)[0]
+    case(f"largest sides {w}x{h}")
+    gk, gv, gt = r.debug_pairs(v, w, h)
+    ok, ov, ot = cpu.pairs(v, w, h)
+    assert np.array_equal(gt, ot) and np.array_equal(gk, ok) and np.array_equal(gv, ov)
+    img, st = r.render(v, w, h, BG)
+    ref, rst = cpu.render(v, w, h, BG)
+    assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
+    assert np.array_equal(img.pixels, ref)
+    r.close()
