@@ -124,7 +124,9 @@ int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height
               const float background[3], uint8_t *rgb_out, sp_stats *stats);
 
 /* As sp_render, but `rgb_out_device` is device memory (width*height*3 bytes)
- * and no image is copied to the host. Returns when the frame is complete. */
+ * and no image is copied to the host. Returns when the frame is complete.
+ * The handle works on its own stream: work other streams queued on the buffer
+ * must be complete before the call (as for every device buffer passed in). */
 int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
                      const float background[3], uint8_t *rgb_out_device, sp_stats *stats);
 

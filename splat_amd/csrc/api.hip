@@ -644,6 +644,15 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
         h->pair_buf = pb;
+        // Pair counts saturate at 2^32 - 1 (the scans never wrap): more pairs
+        // than a u32 index addresses cannot be rendered (the reference's CPU
+        // backend stops there too, src/render/cpu.rs:158-160).
+        const bool binned_first = mode != BL_DIFF;
+        const bool split_on = mode == BL_RGB && h->split_shift > 0;
+        if (h->h_counters[sp::CNT_PAIRS] == 0xFFFFFFFFu ||
+            (binned_first && h->h_counters[sp::CNT_SBPAIRS] == 0xFFFFFFFFu) ||
+            (split_on && h->h_counters[sp::CNT_PAIRS2] == 0xFFFFFFFFu))
+            return fail(SP_E_CAPACITY, "more than 2^32 (gaussian, tile) pairs in one phase of the frame");
         if (h->h_counters[sp::CNT_OVERFLOW] && attempt < 3) {
             uint64_t p = h->h_counters[sp::CNT_PAIRS];
             if (mode == BL_RGB && h->split_shift > 0) p = std::max<uint64_t>(p, h->h_counters[sp::CNT_PAIRS2]);

@@ -15,21 +15,54 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;  // entries per thread per chunk
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
 
+// Block-wide exclusive scan of `v` that also returns the block's float sum of
+// `f` (one barrier round for both).
+__device__ __forceinline__ uint32_t block_scan_fsum(uint32_t v, float f, uint32_t *total, float *ftotal) {
+    constexpr int NW = SCAN_THREADS / 64;
+    __shared__ uint32_t s_w[NW];
+    __shared__ float s_f[NW];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    const float fw = wave_total_f(f);  // in lane 63
+    if (lane_id() == 63) {
+        s_w[w] = inc;
+        s_f[w] = fw;
+    }
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    float ft = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t t = s_w[k];
+        pre += (uint32_t)k < w ? t : 0u;
+        tot += t;
+        ft += s_f[k];
+    }
+    __syncthreads();
+    *total = tot;
+    *ftotal = ft;
+    return pre + inc - v;
+}
+
 // Block-wide exclusive scan of SCAN_ITEMS consecutive entries per thread of
 // `row` starting at `off` (entries beyond `len` read as 0); returns the chunk
-// total. Adds `carry` to every written prefix.
+// total. Adds `carry` to every written prefix. With ftotal, also the chunk's
+// float sum (for telling a total of 2^32 or more from its wrapped u32 sum).
 __device__ __forceinline__ uint32_t scan_chunk(uint32_t *row, uint64_t off, uint64_t len, uint32_t carry,
-                                               bool write) {
+                                               bool write, float *ftotal = nullptr) {
     uint32_t v[SCAN_ITEMS];
     uint32_t sum = 0;
+    float fs = 0.0f;
     const uint64_t b = off + (uint64_t)threadIdx.x * SCAN_ITEMS;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         v[k] = (b + k < len) ? row[b + k] : 0u;
         sum += v[k];
+        if (ftotal) fs += (float)v[k];
     }
     uint32_t total;
-    uint32_t pre = block_exclusive_scan<SCAN_THREADS>(sum, &total) + carry;
+    uint32_t pre = (ftotal ? block_scan_fsum(sum, fs, &total, ftotal) : block_exclusive_scan<SCAN_THREADS>(sum, &total)) +
+                   carry;
     if (write) {
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -54,7 +87,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows(uint32_t *__restrict
 }
 
 // Up to SCAN_MULTI arrays of `rows` rows each, scanned like k_scan_rows in
-// one launch: block b takes row b % rows of array b / rows.
+// one launch: block b takes row b % rows of array b / rows. A row total of
+// 2^32 or more is written as 0xFFFFFFFF (saturated): pair counts of a frame
+// can exceed 32 bits, and a wrapped count would pass for a small one. The
+// exact total is C + k * 2^32, with C the u32 sum and k read off a float sum
+// carried through the same scan (exact while the float's error stays below
+// 2^31, i.e. for totals up to 2^36; beyond that the row is saturated anyway).
+// The in-row prefixes of a saturated row wrap; the capacity clamps downstream
+// keep every access in bounds, and the host rejects the frame.
 constexpr int SCAN_MULTI = 3;
 struct ScanArrays {
     uint32_t *data[SCAN_MULTI];
@@ -64,8 +104,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, 
     const uint32_t k = blockIdx.x / rows, r = blockIdx.x % rows;
     uint32_t *row = a.data[k] + (uint64_t)r * row_len;
     uint32_t carry = 0;
-    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) carry += scan_chunk(row, off, row_len, carry, true);
-    if (a.row_total[k] && threadIdx.x == 0) a.row_total[k][r] = carry;
+    float fsum = 0.0f;
+    for (uint64_t off = 0; off < row_len; off += SCAN_CHUNK) {
+        float fc;
+        carry += scan_chunk(row, off, row_len, carry, true, &fc);
+        fsum += fc;
+    }
+    if (a.row_total[k] && threadIdx.x == 0) {
+        const double wraps = rint(((double)fsum - (double)carry) * (1.0 / 4294967296.0));
+        a.row_total[k][r] = (fsum >= 68719476736.0f || wraps >= 1.0) ? 0xFFFFFFFFu : carry;
+    }
 }
 
 // Radix digit rows: row r = (segment g, digit d) of `row_len` per-column

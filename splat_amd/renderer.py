@@ -135,12 +135,15 @@ class HipRenderer:
         return cells, RenderStats.from_c(st)
 
     def downsample_device(self, src_ptr: int, width: int, height: int, factor: int, dst_ptr: int) -> None:
-        """term::downsample of a device RGB8 frame into device memory (sp_downsample_device)."""
+        """term::downsample of a device RGB8 frame into device memory (sp_downsample_device).
+        Both buffers must be complete on other streams before the call."""
         N.check(self._lib.sp_downsample_device(self._h, C.c_void_p(src_ptr), width, height, factor,
                                                C.c_void_p(dst_ptr)))
 
     def render_device(self, view: N.SpView, width: int, height: int, background, out_ptr: int) -> RenderStats:
-        """Render into device memory at `out_ptr` (width*height*3 bytes); no host copy."""
+        """Render into device memory at `out_ptr` (width*height*3 bytes); no host copy.
+        The library works on its own stream: work that other streams (torch's)
+        queued on the buffer must be complete before the call."""
         st = N.SpStats()
         N.check(self._lib.sp_render_device(self._h, C.byref(view), width, height, N.fptr(_bg(background)),
                                            C.c_void_p(out_ptr), C.byref(st)))

@@ -98,6 +98,7 @@ def test_batch_on_device_outputs():
     views, sizes = S.random_views(s, 5, seed=7, sizes=((200, 100), (64, 64)))
     r = S.HipRenderer(s)
     outs = [torch.zeros(w * h * 3, dtype=torch.uint8, device="cuda") for (w, h) in sizes]
+    torch.cuda.synchronize()  # the zero fills (torch's stream) before the library's writes
     case("batch into device buffers")
     r.render_batch(views, sizes, (0, 0, 0), [o.data_ptr() for o in outs], on_device=True)
     torch.cuda.synchronize()
@@ -159,6 +160,7 @@ def test_full_size_batch_equals_single_views():
     views, sizes = S.random_views(s, 64, seed=3)
     r = S.HipRenderer(s)
     outs = [torch.zeros(w * h * 3, dtype=torch.uint8, device="cuda") for (w, h) in sizes]
+    torch.cuda.synchronize()  # the zero fills (torch's stream) before the library's writes
     case("full-size batch of 64 views")
     r.render_batch(views, sizes, (0, 0, 0), [o.data_ptr() for o in outs], on_device=True)
     one = torch.zeros(max(w * h for w, h in sizes) * 3, dtype=torch.uint8, device="cuda")

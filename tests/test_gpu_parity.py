@@ -298,3 +298,36 @@ def test_largest_sides_and_4k(w, h, oracle):
     assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
     assert np.array_equal(img.pixels, ref)
     r.close()
+
+
+def test_more_than_2_32_pairs(oracle):
+    """70k screen-filling splats over the 65536 tiles of a 4096x4096 frame: 4.6e9
+    (splat, tile) pairs (src/render/gpu.rs:1253-1289 builds the same case). The
+    front-to-back split renders it: the first phase's pairs fit in 32 bits, every
+    pixel saturates within the nearest splats, and the overlaps statistic
+    saturates at 2^32 - 1, as the reference's gpu backend reports it. The image
+    equals the cpu restatement's render of the nearest 64 splats. Without the
+    split the frame's pairs exceed 32 bits: an error (the reference's cpu
+    backend stops there, src/render/cpu.rs:158-160), never a wrapped count."""
+    n = 70_000
+    items = [((0.0, 0.0, 1.0 + i * 1e-4), 50.0, 0.9, (1.0, 0.0, 0.0) if i == 0 else (0.0, 1.0, 0.0))
+             for i in range(n)]
+    s = S.scene_of(items)
+    v = S.view_at((0.0, 0.0, 0.0), 50.0, 0.01)
+    r = S.HipRenderer(s)
+    case("2^32 pairs: split render")
+    img, st = r.render(v, 4096, 4096, (0, 0, 0))
+    assert st.visible == n
+    assert st.overlaps == 2 ** 32 - 1, st.overlaps
+    ref, _ = oracle.CpuTiled(s.subset(np.arange(64))).render(v, 4096, 4096, (0, 0, 0))
+    assert np.array_equal(img.pixels, ref)
+    c = img.get(2048, 2048)
+    assert c[0] > 200 and c[1] < 60 and c[2] == 0, c
+    again, _ = r.render(v, 4096, 4096, (0, 0, 0))
+    assert np.array_equal(again.pixels, img.pixels)
+    case("2^32 pairs: no split is an error")
+    r.set_tuning(blend_batch=256, split_shift=0, split_kmin=0)
+    with pytest.raises(N.SplatError) as e:
+        r.render(v, 4096, 4096, (0, 0, 0))
+    assert e.value.code == N.SP_E_CAPACITY and "2^32" in str(e.value)
+    r.close()

@@ -114,6 +114,7 @@ def test_downsample_device_matches_oracle():
             ff = max(f, 1)
             want = T.downsample(src, f)
             d_dst = torch.full((max(h // ff, 1), max(w // ff, 1), 3), 7, dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()  # torch's fills run on its own stream; the library uses its own
             print(f"[case] downsample {w}x{h} factor {f}", flush=True)
             r.downsample_device(d_src.data_ptr(), w, h, f, d_dst.data_ptr())
             got = d_dst.cpu().numpy()
