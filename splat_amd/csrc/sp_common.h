@@ -37,9 +37,9 @@ constexpr uint64_t RADIX_SMALL_MAX = 16ull << 20;
 __host__ __device__ constexpr uint32_t radix_tile_keys(uint32_t nseg, uint64_t n_cap) {
     return (uint32_t)(RADIX_THREADS * ((uint64_t)nseg * n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
 }
-constexpr int RADIX_BINS = 256;       // 8-bit digits
-// A pass may carry a 9-bit digit (512 bins) when that saves a pass (a 9-bit
-// sort in one pass instead of 8 + 1); tables are sized for it.
+// Radix digits are 8 bits (256 bins); a pass may carry a 9-bit digit (512
+// bins) when that saves a pass (a 9-bit sort in one pass instead of 8 + 1),
+// so tables are sized for 512.
 constexpr int RADIX_BINS_MAX = 512;
 constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
