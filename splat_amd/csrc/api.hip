@@ -496,12 +496,15 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     auto mark = [&](int k) { return events ? hipEventRecord(h->ev[k], s) : hipSuccess; };
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
     HIP_TRY(mark(0));
-    // The uncompacted depth keys live in the depth sort's second buffer until the sort needs it.
-    HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_sbox, h->d_dkey[1], h->d_bcnt, vis,
-                               h->d_dkey[0],
-                               h->d_dval[0], nullptr, nullptr, nullptr));
+    // Depth keys of every (view, splat), FULL_CULLED for the culled ones: the
+    // depth sort's sparse first pass compacts them (scene-index order within a
+    // digit, so the sort stays stable) and writes each view's visible count.
+    HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_sbox, h->d_dkey[0], nullptr, nullptr,
+                               nullptr));
+    if (h->n == 0) HIP_TRY(sp::launch_fill(s, vis, nviews, 0u));  // no sort runs to count them
     int di = 0;
-    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di));
+    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di,
+                                  true));
     h->depth_buf = di;
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
     uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
@@ -1001,8 +1004,8 @@ int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     if (e == hipSuccess) e = hipMemsetAsync(d_bb, 0, n * 8, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h->d_views, h->h_views, sizeof(ViewParams), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
-        e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_sbox, h->d_dkey[1], h->d_bcnt,
-                               h->d_vstat, h->d_dkey[0], h->d_dval[0], d_dbg, d_bb, d_valid);
+        e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_sbox, h->d_dkey[0], d_dbg, d_bb,
+                               d_valid);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(proj, d_dbg, n * 48, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(bbox, d_bb, n * 8, hipMemcpyDeviceToHost);

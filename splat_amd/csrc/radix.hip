@@ -18,6 +18,11 @@
 // Grids are bounded and blocks stride over the live tiles (the live key count
 // is read on the device), so sorting a few keys in a large buffer is cheap.
 //
+// A sparse first pass (the depth sort's) reads a full-length key array in
+// which FULL_CULLED marks absent entries, skips those, makes each value from
+// the entry's position, and writes the segment's present count for the later
+// passes: the sort compacts the visible splats as it goes.
+//
 // The rasterizer sorts twice per view: the visible splats by depth bits (32
 // bits, 4 passes; stable, so equal depths keep scene-index order), then the
 // tile pairs by tile id. Depth digits first and tile digits last is an LSD
@@ -41,18 +46,19 @@ constexpr uint32_t RADIX_GRID = 4096;  // bound on the upsweep / downsweep grid
 // `seg` (radix_group consecutive tiles of 64 * NWAVES * ITEMS keys). gps workgroups per segment
 // stride over the segment's live columns, so the grid follows the capacity
 // only up to a bound and idle work follows the live count.
-template <int ITEMS, int BINS>
+template <int ITEMS, int BINS, bool SPARSE>
 __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ seg_counts,
                                                            uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
                                                            uint32_t gps, int shift, int width,
                                                            uint32_t *__restrict__ table) {
+    constexpr bool sparse = SPARSE;
     __shared__ uint32_t s_h[NWAVES][BINS];
     constexpr int DPT = BINS / RADIX_THREADS;  // digits per thread
     const uint32_t dmask = (1u << width) - 1u;
     const uint32_t seg = blockIdx.x / gps;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
+    const uint64_t n = sparse ? n_cap : min((uint64_t)seg_counts[seg], n_cap);
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS, WAVE_KEYS = 64 * ITEMS;
     const uint32_t tiles = radix_tiles(n, TILE), group = radix_group(tiles), cols = (tiles + group - 1) / group;
     const uint32_t *ks = kin + seg * seg_stride;
@@ -70,7 +76,9 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
                 const uint32_t i = w * WAVE_KEYS + k * 64 + lane;
-                dg[k] = i < rem ? (kt[i] >> shift) & dmask : BINS;  // BINS: no key
+                const uint32_t key = i < rem ? kt[i] : FULL_CULLED;
+                const bool present = i < rem && !(sparse && key == FULL_CULLED);
+                dg[k] = present ? (key >> shift) & dmask : BINS;  // BINS: no key
             }
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
@@ -110,12 +118,13 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
 // start in different LDS banks; four large-tile workgroups still fit a CU.
 __device__ __forceinline__ uint32_t ex_slot(uint32_t p) { return p + (p >> 5); }
 
-template <int ITEMS, int BINS>
+template <int ITEMS, int BINS, bool SPARSE>
 __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, const uint32_t *__restrict__ vs,
                                                uint32_t *__restrict__ ko, uint32_t *__restrict__ vo, uint64_t n,
                                                uint32_t t, int shift, int width, uint32_t *run,
                                                uint32_t (*s_wh)[BINS], uint32_t *s_lstart, uint32_t *s_gbase,
                                                uint2 *s_ex) {
+    constexpr bool sparse = SPARSE;
     constexpr int DPT = BINS / RADIX_THREADS;
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS, WAVE_KEYS = 64 * ITEMS;
     const uint64_t base = (uint64_t)t * TILE;
@@ -127,16 +136,19 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     for (uint32_t k = threadIdx.x; k < NWAVES * BINS; k += RADIX_THREADS) (&s_wh[0][0])[k] = 0;
     __syncthreads();
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    bool present[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = w * WAVE_KEYS + k * 64 + lane;
-        const bool valid = i < rem;
-        key[k] = valid ? kt[i] : 0u;
-        val[k] = valid ? vt[i] : 0u;
+        key[k] = i < rem ? kt[i] : 0u;
+        // A sparse pass skips the FULL_CULLED entries and makes the values:
+        // the entry's position in the segment.
+        present[k] = i < rem && !(sparse && key[k] == FULL_CULLED);
+        val[k] = present[k] ? (sparse ? (uint32_t)(base + i) : vt[i]) : 0u;
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const bool valid = w * WAVE_KEYS + k * 64 + lane < rem;
+        const bool valid = present[k];
         const uint32_t d = (key[k] >> shift) & dmask;
         // m = the lanes holding this lane's digit. A wave whose keys share
         // one digit needs no bit tests; otherwise one ballot per bit that the
@@ -193,13 +205,13 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        if (w * WAVE_KEYS + k * 64 + lane < rem) {
+        if (present[k]) {
             const uint32_t dk = (key[k] >> shift) & dmask;
             s_ex[ex_slot(s_lstart[dk] + s_wh[w][dk] + rank[k])] = make_uint2(key[k], val[k]);
         }
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < rem; j += RADIX_THREADS) {
+    for (uint32_t j = threadIdx.x; j < tot; j += RADIX_THREADS) {  // tot: the tile's present keys
         const uint2 e = s_ex[ex_slot(j)];
         const uint32_t pos = s_gbase[(e.x >> shift) & dmask] + j;
         if (pos < n) {  // always true for consistent counts; guards the buffer
@@ -209,16 +221,17 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
     }
 }
 
-template <int ITEMS, int BINS>
+template <int ITEMS, int BINS, bool SPARSE>
 __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__restrict__ kin,
                                                              const uint32_t *__restrict__ vin,
                                                              uint32_t *__restrict__ kout,
                                                              uint32_t *__restrict__ vout,
-                                                             const uint32_t *__restrict__ seg_counts,
+                                                             uint32_t *__restrict__ seg_counts,
                                                              uint64_t seg_stride, uint64_t n_cap, uint32_t bps,
                                                              uint32_t gps, int shift, int width,
                                                              const uint32_t *__restrict__ table,
                                                              const uint32_t *__restrict__ row_total) {
+    constexpr bool sparse = SPARSE;
     __shared__ uint32_t s_wh[NWAVES][BINS];  // per-wave digit counters, then per-wave offsets
     __shared__ uint32_t s_lstart[BINS];      // block-local start of each digit
     __shared__ uint32_t s_gbase[BINS];       // segment position minus local position
@@ -226,7 +239,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
     __shared__ uint2 s_ex[TILE + TILE / 32];       // key/value exchange in digit order (padded)
     const uint32_t seg = blockIdx.x / gps;
-    const uint64_t n = min((uint64_t)seg_counts[seg], n_cap);
+    const uint64_t n = sparse ? n_cap : min((uint64_t)seg_counts[seg], n_cap);
     const uint32_t tiles = radix_tiles(n, TILE), group = radix_group(tiles), cols = (tiles + group - 1) / group;
     if (blockIdx.x % gps >= cols) return;  // block-uniform
     // Segment position of each of this thread's digits' first key: the row
@@ -242,6 +255,9 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
     row_base[0] = block_exclusive_scan<RADIX_THREADS>(rsum, &seg_total);
 #pragma unroll
     for (int j = 1; j < DPT; ++j) row_base[j] = row_base[j - 1] + rt[j - 1];
+    // A sparse pass publishes the segment's present keys: the later passes'
+    // segment length (no block of this pass reads it).
+    if (sparse && blockIdx.x % gps == 0 && threadIdx.x == 0) seg_counts[seg] = seg_total;
     const uint32_t *ks = kin + seg * seg_stride;
     const uint32_t *vs = vin + seg * seg_stride;
     uint32_t *ko = kout + seg * seg_stride;
@@ -253,7 +269,8 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
             run[j] = row_base[j] + table[((uint64_t)seg * BINS + threadIdx.x * DPT + j) * bps + col];
         const uint32_t t_end = min(tiles, (col + 1) * group);
         for (uint32_t t = col * group; t < t_end; ++t) {
-            downsweep_tile<ITEMS, BINS>(ks, vs, ko, vo, n, t, shift, width, run, s_wh, s_lstart, s_gbase, s_ex);
+            downsweep_tile<ITEMS, BINS, SPARSE>(ks, vs, ko, vo, n, t, shift, width, run, s_wh, s_lstart, s_gbase,
+                                                s_ex);
             __syncthreads();  // LDS is reused by the next tile
         }
     }
@@ -287,10 +304,28 @@ int radix_plan(int bits, int *widths) {
 }
 
 namespace {
+// One pass: upsweep, digit-row scan, downsweep from keys[cur] / vals[cur] into the other buffers.
+template <int ITEMS, int BINS, bool SPARSE>
+hipError_t radix_pass(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], int cur, uint32_t *seg_counts,
+                      uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, uint32_t bps, uint32_t gps, int shift,
+                      int width, uint32_t *row_total, uint32_t *table) {
+    constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
+    hipLaunchKernelGGL((k_upsweep<ITEMS, BINS, SPARSE>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                       seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = launch_scan_digit_rows(s, table, bps, nseg, BINS, SPARSE ? nullptr : seg_counts, n_cap, TILE, row_total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_downsweep<ITEMS, BINS, SPARSE>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
+                       vals[cur], keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps, shift, width,
+                       table, row_total);
+    return hipGetLastError();
+}
+
 template <int ITEMS>
-hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
+hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], uint32_t *seg_counts,
                         uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
-                        uint32_t *table, int *out_index) {
+                        uint32_t *table, int *out_index, bool sparse_first) {
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
     int widths[8];
     const int passes = radix_plan(bits, widths);
@@ -303,25 +338,20 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], con
     for (int p = 0; p < passes; ++p) {
         const int width = widths[p];  // digit bits this pass carries
         const uint32_t bins = width > 8 ? 512u : 256u;
+        const bool sparse = sparse_first && p == 0;
+        // One instantiation per (digit bins, sparse first pass); the common
+        // passes carry no sparse code.
+        hipError_t e;
         if (bins == 512)
-            hipLaunchKernelGGL((k_upsweep<ITEMS, 512>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
-                               seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
+            e = sparse ? radix_pass<ITEMS, 512, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps, gps,
+                                                       shift, width, row_total, table)
+                       : radix_pass<ITEMS, 512, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps,
+                                                        gps, shift, width, row_total, table);
         else
-            hipLaunchKernelGGL((k_upsweep<ITEMS, 256>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
-                               seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess)
-            e = launch_scan_digit_rows(s, table, bps, nseg, bins, seg_counts, n_cap, TILE, row_total);
-        if (e != hipSuccess) return e;
-        if (bins == 512)
-            hipLaunchKernelGGL((k_downsweep<ITEMS, 512>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
-                               vals[cur], keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps,
-                               shift, width, table, row_total);
-        else
-            hipLaunchKernelGGL((k_downsweep<ITEMS, 256>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
-                               vals[cur], keys[1 - cur], vals[1 - cur], seg_counts, seg_stride, n_cap, bps, gps,
-                               shift, width, table, row_total);
-        e = hipGetLastError();
+            e = sparse ? radix_pass<ITEMS, 256, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps, gps,
+                                                       shift, width, row_total, table)
+                       : radix_pass<ITEMS, 256, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps,
+                                                        gps, shift, width, row_total, table);
         if (e != hipSuccess) return e;
         shift += width;
         cur = 1 - cur;
@@ -331,16 +361,16 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], con
 }
 }  // namespace
 
-hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
+hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], uint32_t *seg_counts,
                              uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
-                             uint32_t *table, int *out_index) {
+                             uint32_t *table, int *out_index, bool sparse_first) {
     *out_index = 0;
     if (bits <= 0 || n_cap == 0 || nseg == 0) return hipSuccess;
     if (radix_tile_keys(nseg, n_cap) == RADIX_THREADS * RADIX_ITEMS_SMALL)
         return radix_passes<RADIX_ITEMS_SMALL>(s, keys, vals, seg_counts, nseg, seg_stride, n_cap, bits, row_total,
-                                               table, out_index);
+                                               table, out_index, sparse_first);
     return radix_passes<RADIX_ITEMS_LARGE>(s, keys, vals, seg_counts, nseg, seg_stride, n_cap, bits, row_total,
-                                           table, out_index);
+                                           table, out_index, sparse_first);
 }
 
 }  // namespace sp

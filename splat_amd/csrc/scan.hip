@@ -128,7 +128,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__re
                                                                   uint64_t n_cap, uint32_t tile_keys,
                                                                   uint32_t *__restrict__ row_total) {
     const uint32_t g = blockIdx.x / bins;
-    const uint64_t n = min((uint64_t)seg_counts[g], n_cap);
+    const uint64_t n = seg_counts ? min((uint64_t)seg_counts[g], n_cap) : n_cap;  // none: every segment holds n_cap
     const uint64_t live = min(row_len, (uint64_t)radix_columns(n, tile_keys));
     uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
     uint32_t carry = 0;

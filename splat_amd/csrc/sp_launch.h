@@ -18,25 +18,24 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
                                   uint64_t row_len, uint32_t rows);
 // Radix tables: rows (segment g, digit d < bins); only the live prefix of each row
-// (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries) is scanned,
-// in place and from 0; row_total[g * 256 + d] receives the row's sum.
+// (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries, n_cap when
+// seg_counts is null) is scanned, in place and from 0; row_total[g * bins + d]
+// receives the row's sum.
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg, uint32_t bins,
                                   const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
                                   uint32_t *row_total);
 // p[0..n) = value (a kernel, so it may be captured in a graph).
 hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 
-// ---- project.hip: (a) projection + SH colour + per-view compaction ----
+// ---- project.hip: (a) projection + SH colour ----
 uint32_t project_blocks(uint32_t n);
 // Projects every splat into each of `nviews` views: records rec[3*(v*n+i)..]
-// for visible splats and their pixel boxes sbox[v*n + i]; full[v*n + i] depth bits or FULL_CULLED; bcnt (nviews
-// rows of project_blocks(n) block counts, scanned in place); vis_count[v];
-// compacted depth keys dkey[v*n + j] with scene indices dval[v*n + j] for
-// j < vis_count[v], in scene-index order. dbg / dbg_bbox / dbg_valid
-// (optional, n entries) get view 0's full per-splat projection.
+// for visible splats, their pixel boxes sbox[v*n + i], and full[v*n + i] =
+// the depth bits, or FULL_CULLED for a culled splat (the depth sort's sparse
+// first pass reads it directly). dbg / dbg_bbox / dbg_valid (optional, n
+// entries) get view 0's full per-splat projection.
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
-                          uint2 *sbox, uint32_t *full, uint32_t *bcnt, uint32_t *vis_count, uint32_t *dkey,
-                          uint32_t *dval, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
+                          uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
 // Words of the per-block digit-count table for nseg segments of up to n_cap
@@ -53,9 +52,13 @@ int radix_plan(int bits, int *widths);
 // at offset g * seg_stride (device counts; n_cap sizes the grid). Ping-pongs
 // between keys[0]/keys[1]; *out_index names the buffer with the result.
 // row_total holds nseg * RADIX_BINS_MAX u32, table radix_table_words(nseg, n_cap) u32.
-hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], const uint32_t *seg_counts,
+// sparse_first: every segment holds n_cap entries in keys[0], of which those
+// equal to FULL_CULLED are absent; the first pass sorts the others with values
+// made from their segment positions (vals[0] is not read) and writes each
+// segment's count of them into seg_counts, which the later passes read.
+hipError_t launch_radix_sort(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], uint32_t *seg_counts,
                              uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
-                             uint32_t *table, int *out_index);
+                             uint32_t *table, int *out_index, bool sparse_first = false);
 
 // ---- tiles.hip: (b) emission, (d) ranges ----
 uint32_t emit_blocks(uint32_t n);
