@@ -160,12 +160,14 @@ def main() -> None:
     # gathers its 48-byte record; tile bounds read; the RGB8 frame written.
     blend_bytes = staged * (IDX_BYTES + REC_BYTES) + (tiles + 1) * 4 + W * H * 3
     achieved = blend_bytes / (blend_ms * 1e-3) / 1e9 if blend_ms > 0 else 0.0
+    # Measured HBM bytes of both blend dispatches of a view: copy-calibrated
+    # FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc runs of this same workload.
     traffic = None
-    pmc = ROOT / "profiles" / "blend_1m_1080p_pmc.json"
+    pmc = ROOT / "profiles" / "plain_1m_1080p_pmc.json"
     if pmc.exists():
         try:
             traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
+        except (OSError, ValueError):
             traffic = None
     # Whole-view compulsory bytes (inputs once, records once, pairs once, frame once).
     view_bytes = (args.splats * SPLAT_IN_BYTES[3] + visible * REC_BYTES + pairs * (8 + REC_BYTES + IDX_BYTES)
