@@ -139,8 +139,12 @@ struct sp_handle {
     uint2 *d_box = nullptr;  // depth-ordered pixel boxes, view_cap x n
     uint2 *d_sbox = nullptr; // scene-order pixel boxes written by the projection, view_cap x n
     uint32_t *d_dkey[2] = {nullptr, nullptr}, *d_dval[2] = {nullptr, nullptr};
-    uint32_t *d_vstat = nullptr;  // 5 x view_cap: visible, pairs, emission view bases, superblock-pair
-                                  // bases, unfinished tiles after the first phase
+    // One allocation: CNT_SLOTS frame counters (d_counters), then 5 x view_cap
+    // view stats (d_vstat): visible, pairs, emission view bases, superblock-pair
+    // bases, unfinished tiles after the first phase. The counters, visible
+    // counts and pair counts are thus contiguous and come back in one copy.
+    uint32_t *d_statblk = nullptr;
+    uint32_t *d_vstat = nullptr;
     uint32_t *d_bcnt = nullptr;   // view_cap x block counts (projection, emission)
     uint32_t *d_bcnt_all = nullptr;  // view_cap x block counts of the unsplit emission
     uint32_t *d_bsb = nullptr;    // view_cap x block counts of superblock pairs (binned first phase)
@@ -161,7 +165,7 @@ struct sp_handle {
     uint8_t *d_rgb = nullptr;
     uint64_t rgb_cap = 0;
     uint32_t *d_counters = nullptr;
-    uint32_t *h_counters = nullptr;  // pinned: CNT_SLOTS counters then 2*view_cap view stats
+    uint32_t *h_counters = nullptr;  // pinned copy of d_statblk's first CNT_SLOTS + 2 * view_cap words
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     sp_frame_info last{};
     int pair_buf = 0;
@@ -308,7 +312,8 @@ int ensure_views(sp_handle *h, uint32_t views) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
     }
-    dfree(h, h->d_vstat, old * 20);
+    dfree(h, h->d_statblk, sp::CNT_SLOTS * 4 + old * 20);
+    h->d_vstat = h->d_counters = nullptr;
     dfree(h, h->d_bcnt, old * blocks * 4);
     dfree(h, h->d_bcnt_all, old * blocks * 4);
     dfree(h, h->d_bsb, old * blocks * 4);
@@ -327,7 +332,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
     }
-    if (!rc) rc = dalloc(h, (void **)&h->d_vstat, v * 20);
+    if (!rc) rc = dalloc(h, (void **)&h->d_statblk, sp::CNT_SLOTS * 4 + v * 20);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bcnt_all, v * blocks * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_bsb, v * blocks * 4);
@@ -337,6 +342,13 @@ int ensure_views(sp_handle *h, uint32_t views) {
     if (hipHostMalloc((void **)&h->h_views, v * sizeof(ViewParams), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&h->h_counters, (sp::CNT_SLOTS + 2 * v) * 4, hipHostMallocDefault) != hipSuccess)
         return fail(SP_E_NOMEM, "hipHostMalloc failed");
+    // The frame counters start at 0.
+    h->d_counters = h->d_statblk;
+    h->d_vstat = h->d_statblk + sp::CNT_SLOTS;
+    const uint32_t zero[sp::CNT_SLOTS] = {0};
+    if (hipMemcpyAsync(h->d_counters, zero, sizeof zero, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(SP_E_DEVICE, "counter init failed");
     h->view_cap = views;
     return ensure_table(h);
 }
@@ -563,9 +575,9 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
                                  nullptr, h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
                                  h->blend_batch));
     HIP_TRY(mark(6));
-    HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, sp::CNT_SLOTS * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS, vis, nviews * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h->h_counters + sp::CNT_SLOTS + nviews, view_pairs, nviews * 4, hipMemcpyDeviceToHost, s));
+    // Counters, visible counts and pair counts (view_cap apart) in one copy.
+    HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, (sp::CNT_SLOTS + h->view_cap + nviews) * 4,
+                           hipMemcpyDeviceToHost, s));
     *pair_buf = pi;
     return SP_OK;
 }
@@ -676,7 +688,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     }
     const double total_ms = now_ms() - t0;
     const uint32_t *vis = h->h_counters + sp::CNT_SLOTS;
-    const uint32_t *vpairs = vis + nviews;
+    const uint32_t *vpairs = vis + h->view_cap;
     uint64_t vis_sum = 0;
     for (uint32_t v = 0; v < nviews; ++v) {
         vis_sum += vis[v];
@@ -777,13 +789,7 @@ int open_handle(sp_handle *h) {
 
 // Counters, one view's arrays and the initial pair capacity.
 int finish_handle(sp_handle *h) {
-    int rc = dalloc(h, (void **)&h->d_counters, sp::CNT_SLOTS * 4);
-    if (rc) return rc;
-    const uint32_t zero[sp::CNT_SLOTS] = {0};
-    if (hipMemcpyAsync(h->d_counters, zero, sizeof zero, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-        return fail(SP_E_DEVICE, "counter init failed");
-    rc = ensure_views(h, 1);
+    int rc = ensure_views(h, 1);
     // The split plain path materialises only the first phase's pairs at once.
     const uint64_t cap0 = h->opt.initial_pairs ? h->opt.initial_pairs
                                                : std::max<uint64_t>((h->diff ? 16ull : 4ull) * h->n, 1ull << 20);
@@ -829,7 +835,7 @@ void sp_destroy(sp_handle *h) {
         (void)hipFree(h->d_pkey[b]);
         (void)hipFree(h->d_pval[b]);
     }
-    (void)hipFree(h->d_vstat);
+    (void)hipFree(h->d_statblk);
     (void)hipFree(h->d_bcnt);
     (void)hipFree(h->d_bcnt_all);
     (void)hipFree(h->d_views);
@@ -842,7 +848,6 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_trow);
     (void)hipFree(h->d_bsb);
     (void)hipFree(h->d_rgb);
-    (void)hipFree(h->d_counters);
     (void)hipFree(h->d_rowtot);
     (void)hipFree(h->d_powner);
     (void)hipFree(h->d_pixT);

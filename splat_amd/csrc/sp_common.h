@@ -45,7 +45,7 @@ constexpr uint32_t MAX_VIEWS = 4096;  // views per batch
 
 // Device counter slots (uint32 each) in the per-handle counter block.
 enum Counter : int {
-    CNT_PAIRS = 1,     // pairs of the (first) emission phase, exact even when over capacity
+    CNT_PAIRS = 1,     // pairs of the (first) emission phase, exact even when over capacity (0xFFFFFFFF: 2^32 or more)
     CNT_OVERFLOW = 2,  // 1 when an emission phase exceeded the pair capacity
     CNT_STAGED = 3,    // pair records the blends gathered (early termination skips the rest)
     CNT_PAIRS2 = 4,    // pairs of the second emission phase

@@ -69,7 +69,7 @@ uint32_t emit_blocks(uint32_t n);
 // = 0 skips view v). FIRST clears vopen (when given) for the row prefixes.
 // bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
 // entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
-// (SECOND), exact, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
+// (SECOND), saturated at 0xFFFFFFFF, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
 // also writes each view's unsplit pair count to view_pairs[v]. With powner
 // set, pval[e] = e and powner[e] = record index instead.
 // box: n entries per view, gathered by FIRST from the scene-order boxes sbox (the
