@@ -60,6 +60,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t n = sparse ? n_cap : min((uint64_t)seg_counts[seg], n_cap);
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS, WAVE_KEYS = 64 * ITEMS;
+    static_assert(ITEMS % 4 == 0, "whole-slice loads take four keys per lane");
     const uint32_t tiles = radix_tiles(n, TILE), group = radix_group(tiles), cols = (tiles + group - 1) / group;
     const uint32_t *ks = kin + seg * seg_stride;
     for (uint32_t col = blockIdx.x % gps; col < cols; col += gps) {
@@ -69,16 +70,39 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
         for (uint32_t t = col * group; t < t_end; ++t) {
             const uint64_t base = (uint64_t)t * TILE;
             const uint32_t rem = (uint32_t)min((uint64_t)TILE, n - base);
-            const uint32_t *kt = ks + base;
-            // All of the wave's loads are issued before the first count, so each
-            // wave keeps ITEMS coalesced 256-byte reads in flight.
+            // This wave's slice of the tile: keys [w * WAVE_KEYS, +WAVE_KEYS),
+            // of which wrem are in range. All of the wave's loads are issued
+            // before the first count. Counting ignores key order, so a full,
+            // 16-byte aligned slice loads as ITEMS / 4 contiguous 1 KB reads of
+            // four keys per lane; other slices load one key per lane per read.
+            const uint32_t *kw = ks + base + w * WAVE_KEYS;
+            const uint32_t wrem = rem > w * WAVE_KEYS ? min(rem - w * WAVE_KEYS, WAVE_KEYS) : 0u;
+            uint32_t key[ITEMS];
+            bool in[ITEMS];
+            if (wrem == WAVE_KEYS && (reinterpret_cast<uintptr_t>(kw) & 15u) == 0) {  // wave-uniform
+#pragma unroll
+                for (int q = 0; q < ITEMS / 4; ++q) {
+                    const uint4 v = reinterpret_cast<const uint4 *>(kw)[q * 64 + lane];
+                    key[4 * q] = v.x;
+                    key[4 * q + 1] = v.y;
+                    key[4 * q + 2] = v.z;
+                    key[4 * q + 3] = v.w;
+                }
+#pragma unroll
+                for (int k = 0; k < ITEMS; ++k) in[k] = true;
+            } else {
+#pragma unroll
+                for (int k = 0; k < ITEMS; ++k) {
+                    const uint32_t i = k * 64 + lane;
+                    in[k] = i < wrem;
+                    key[k] = in[k] ? kw[i] : FULL_CULLED;
+                }
+            }
             uint32_t dg[ITEMS];
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {
-                const uint32_t i = w * WAVE_KEYS + k * 64 + lane;
-                const uint32_t key = i < rem ? kt[i] : FULL_CULLED;
-                const bool present = i < rem && !(sparse && key == FULL_CULLED);
-                dg[k] = present ? (key >> shift) & dmask : BINS;  // BINS: no key
+                const bool present = in[k] && !(sparse && key[k] == FULL_CULLED);
+                dg[k] = present ? (key[k] >> shift) & dmask : BINS;  // BINS: no key
             }
 #pragma unroll
             for (int k = 0; k < ITEMS; ++k) {

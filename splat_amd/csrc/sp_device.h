@@ -188,12 +188,17 @@ __device__ __forceinline__ uint32_t sat_u32(float v) {
 // reference's Cephes-style range reduction and degree-6 polynomial with
 // unfused multiply-adds (self + a*b). Bit-identical to the CPU blend when
 // compiled without contraction.
+// The first reduction step x - n * C1 is one fused multiply-add: C1 =
+// 0.693359375 = 355 / 2^9 has 9 significant bits and n is an integer of at
+// most 8 bits here (x >= -87 gives n >= -126, and the callers' inputs are at
+// most about 0), so n * C1 is exact and rounding x - n * C1 once is what the
+// unfused form does.
 __device__ __forceinline__ float exp_neg(float xin) {
     // max(xin, -87) as one median-of-three (fmaxf adds a canonicalising max
     // in IEEE mode); equal for every non-NaN input, and inputs are finite.
     const float x = __builtin_amdgcn_fmed3f(xin, -87.0f, 3.402823466e38f);
     const float n = rintf(x * 1.44269504088896340736f);
-    const float f = (x - n * 0.6933594f) - n * -2.1219444e-4f;
+    const float f = fmaf(-n, 0.6933594f, x) - n * -2.1219444e-4f;
     float p = 1.9875691e-4f;
     p = 1.3981999e-3f + p * f;
     p = 8.333452e-3f + p * f;
@@ -214,7 +219,7 @@ __device__ __forceinline__ float exp_neg(float xin) {
 __device__ __forceinline__ float exp_neg_times(float xin, float op) {
     const float x = __builtin_amdgcn_fmed3f(xin, -87.0f, 3.402823466e38f);
     const float n = rintf(x * 1.44269504088896340736f);
-    const float f = (x - n * 0.6933594f) - n * -2.1219444e-4f;
+    const float f = fmaf(-n, 0.6933594f, x) - n * -2.1219444e-4f;
     float p = 1.9875691e-4f;
     p = 1.3981999e-3f + p * f;
     p = 8.333452e-3f + p * f;
