@@ -190,7 +190,7 @@ struct sp_handle {
     uint32_t *d_stamp = nullptr;  // gpair_cap: id of the backward that last wrote the pair's gradients
     uint32_t stamp_id = 0;
     uint64_t gpair_cap = 0;
-    float *d_g2d = nullptr;  // NG2D x g2d_cap per-record gradients
+    float *d_g2d = nullptr;  // g2d_cap x NG2D per-record gradients, record-major
     uint64_t g2d_cap = 0;
     float *d_part = nullptr;  // segmented-reduction carries
     uint64_t part_cap = 0;
@@ -1310,13 +1310,13 @@ int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *
         HIP_TRY(sp::launch_fill(s, h->d_stamp, h->gpair_cap, 0u));
         h->stamp_id = 1;
     }
-    HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, h->g2d_cap * sp::NG2D, 0.0f));
+    HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, (uint64_t)nv * h->n * sp::NG2D, 0.0f));  // the records in use
     HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->d_tile_start, h->d_pval[h->pair_buf],
                                  h->d_powner, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
                                  h->d_stamp, h->stamp_id));
     HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_stamp, h->stamp_id, h->d_powner, np, h->d_g2d,
-                                  h->g2d_cap, h->d_part));
-    HIP_TRY(sp::launch_project_bwd(s, dev_scene(h), h->d_views, nv, h->d_g2d, h->g2d_cap, grads->means,
+                                  h->d_part));
+    HIP_TRY(sp::launch_project_bwd(s, dev_scene(h), h->d_views, nv, h->d_g2d, grads->means,
                                    grads->log_scales, grads->quats, grads->opacity_logits, grads->colors,
                                    grads->sh_rest));
     HIP_TRY(hipStreamSynchronize(s));

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
 __global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restrict__ gpair, uint64_t gstride,
                                                            const uint32_t *__restrict__ stamp, uint32_t stamp_id,
                                                            const uint32_t *__restrict__ powner, uint64_t np,
-                                                           float *__restrict__ g2d, uint64_t g2d_stride,
+                                                           float *__restrict__ g2d,
                                                            float *__restrict__ part) {
     constexpr int NW = SR_THREADS / 64;
     __shared__ float s_v[NG2D][SR_THREADS];
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restri
     const bool after = u == cnt && base + cnt < np && powner[base + cnt] == o;
     if (!before && !after) {
 #pragma unroll
-        for (int c = 0; c < NG2D; ++c) g2d[c * g2d_stride + o] = sum[c];
+        for (int c = 0; c < NG2D; ++c) g2d[(uint64_t)o * NG2D + c] = sum[c];
     } else {
         float *dst = part + ((uint64_t)blockIdx.x * 2 + (before ? 0 : 1)) * NG2D;
 #pragma unroll
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restri
 // adds its tail partial and the head partials of the blocks it runs into.
 __global__ __launch_bounds__(256) void k_seg_carry(const uint32_t *__restrict__ powner, uint64_t np,
                                                    uint32_t nblocks, const float *__restrict__ part,
-                                                   float *__restrict__ g2d, uint64_t g2d_stride) {
+                                                   float *__restrict__ g2d) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b >= nblocks) return;
     const uint64_t base = (uint64_t)b * SR_THREADS;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void k_seg_carry(const uint32_t *__restrict__ 
         if (powner[end2 - 1] != o || end2 >= np || powner[end2] != o) break;
     }
 #pragma unroll
-    for (int c = 0; c < NG2D; ++c) g2d[c * g2d_stride + o] = sum[c];
+    for (int c = 0; c < NG2D; ++c) g2d[(uint64_t)o * NG2D + c] = sum[c];
 }
 
 // Real SH basis of degree DEG (the order sh_color evaluates it) with its
@@ -355,7 +355,7 @@ struct GradOut {
 template <int DEG>
 __global__ __launch_bounds__(PBWD_THREADS) void k_project_bwd(DevScene sc, const ViewParams *__restrict__ views,
                                                               uint32_t nviews, const float *__restrict__ g2d,
-                                                              uint64_t g2d_stride, GradOut go) {
+                                                              GradOut go) {
     constexpr int R = (DEG + 1) * (DEG + 1) - 1;
     const uint32_t i = blockIdx.x * PBWD_THREADS + threadIdx.x;
     if (i >= sc.n) return;
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(PBWD_THREADS) void k_project_bwd(DevScene sc, const
         bool any = false;
 #pragma unroll
         for (int c = 0; c < NG2D; ++c) {
-            g[c] = g2d[c * g2d_stride + rid];
+            g[c] = g2d[rid * NG2D + c];
             any = any || g[c] != 0.0f;
         }
         if (!any) continue;
@@ -551,10 +551,9 @@ __global__ __launch_bounds__(PBWD_THREADS) void k_project_bwd(DevScene sc, const
 
 template <int DEG>
 void launch_pbwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, const float *g2d,
-                 uint64_t g2d_stride, const GradOut &go) {
+                 const GradOut &go) {
     const uint32_t grid = (sc.n + PBWD_THREADS - 1) / PBWD_THREADS;
-    hipLaunchKernelGGL(k_project_bwd<DEG>, dim3(grid), dim3(PBWD_THREADS), 0, s, sc, views, nviews, g2d, g2d_stride,
-                       go);
+    hipLaunchKernelGGL(k_project_bwd<DEG>, dim3(grid), dim3(PBWD_THREADS), 0, s, sc, views, nviews, g2d, go);
 }
 
 }  // namespace
@@ -573,29 +572,28 @@ hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nvi
 uint64_t seg_blocks(uint64_t np) { return (np + SR_THREADS - 1) / SR_THREADS; }
 
 hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
-                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, uint64_t g2d_stride,
-                             float *part) {
+                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, float *part) {
     if (np == 0) return hipSuccess;
     const uint64_t nb = seg_blocks(np);
     hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(SR_THREADS), 0, s, gpair, gstride, stamp, stamp_id,
-                       powner, np, g2d, g2d_stride, part);
+                       powner, np, g2d, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_seg_carry, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, powner, np, (uint32_t)nb,
-                       part, g2d, g2d_stride);
+                       part, g2d);
     return hipGetLastError();
 }
 
 hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
-                              const float *g2d, uint64_t g2d_stride, float *g_means, float *g_log_scales,
+                              const float *g2d, float *g_means, float *g_log_scales,
                               float *g_quats, float *g_logits, float *g_colors, float *g_sh_rest) {
     if (sc.n == 0) return hipSuccess;
     const GradOut go{g_means, g_log_scales, g_quats, g_logits, g_colors, g_sh_rest};
     switch (sc.deg) {
-    case 0: launch_pbwd<0>(s, sc, views, nviews, g2d, g2d_stride, go); break;
-    case 1: launch_pbwd<1>(s, sc, views, nviews, g2d, g2d_stride, go); break;
-    case 2: launch_pbwd<2>(s, sc, views, nviews, g2d, g2d_stride, go); break;
-    default: launch_pbwd<3>(s, sc, views, nviews, g2d, g2d_stride, go); break;
+    case 0: launch_pbwd<0>(s, sc, views, nviews, g2d, go); break;
+    case 1: launch_pbwd<1>(s, sc, views, nviews, g2d, go); break;
+    case 2: launch_pbwd<2>(s, sc, views, nviews, g2d, go); break;
+    default: launch_pbwd<3>(s, sc, views, nviews, g2d, go); break;
     }
     return hipGetLastError();
 }

@@ -128,15 +128,15 @@ hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nvi
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id);
 // Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.
 uint64_t seg_blocks(uint64_t np);
-// g2d[c * g2d_stride + record] = sum over the record's pairs of gpair where
-// stamp == stamp_id (others count as zero). Records without pairs are
-// untouched; the caller zero-fills g2d.
+// g2d[record * NG2D + c] = sum over the record's pairs of gpair where
+// stamp == stamp_id (others count as zero). Record-major, so each record's
+// sums are one contiguous 36-byte write (records come in depth order, i.e.
+// scattered). Records without pairs are untouched; the caller zero-fills g2d.
 hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
-                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, uint64_t g2d_stride,
-                             float *part);
+                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, float *part);
 // Per-gaussian parameter gradients (sc.storage == STORE_PARAMS), summed over views.
 hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
-                              const float *g2d, uint64_t g2d_stride, float *g_means, float *g_log_scales,
+                              const float *g2d, float *g_means, float *g_log_scales,
                               float *g_quats, float *g_logits, float *g_colors, float *g_sh_rest);
 // p[0..n) = value for float arrays.
 hipError_t launch_fill_f32(hipStream_t s, float *p, uint64_t n, float value);
