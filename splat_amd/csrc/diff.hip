@@ -18,7 +18,8 @@
 //        dL/dmean2 = dL/dpw * (A dx + B dy, B dx + C dy)
 //        dL/dconic = dL/dpw * (-dx^2 / 2, -dx dy, -dy^2 / 2)
 //      with G = dL/dpixel. The nine values are summed over the tile's 256
-//      pixels in a fixed order (DPP sums within a wave, then the four waves);
+//      pixels in a fixed order (one nine-value halving reduction within a
+//      wave, then the four waves);
 //      the mean and conic terms are summed as moments of dL/dpw (dx, dy,
 //      dx^2, dx dy, dy^2) and multiplied by the conic once per record. They are
 //      stored per pair at its emission index. Records whose falloff cannot
@@ -46,6 +47,55 @@ constexpr int BWD_BATCH = 64;  // records per LDS step of the adjoint blend
 constexpr int SR_THREADS = 256;
 constexpr int PBWD_THREADS = 128;
 
+// Totals over the 64 lanes of the NG2D = 9 per-lane values g[] at once, by
+// recursive halving. Each step pairs lanes that hold the same subset of the
+// values; each lane adds its partner's copy of the whole subset and keeps the
+// half that its lane bit selects, so later steps carry fewer values (9, 5, 3,
+// 2, then 1 per lane): 34 VALU against 54 for nine one-value reductions. The
+// pairs are lane 15 - i (row_mirror; lane bit 3 selects), 7 - i
+// (row_half_mirror; bit 2), i ^ 2 and i ^ 1 (quad_perm; bits 1, 0); each lane
+// of a row of 16 then holds one value's sum over its row, and swaps across
+// rows (permlane16 / permlane32) add the four rows. Lane i < 16 ends with the
+// total of value totals9_index(i) (9 of the 16 lanes hold one). Every lane
+// must be active; the order of additions is fixed.
+template <int CTRL>
+__device__ __forceinline__ float halve_pair(float lo, float hi, bool upper) {
+    const float s_lo = lo + dpp_f<CTRL, 0xF>(lo), s_hi = hi + dpp_f<CTRL, 0xF>(hi);
+    return upper ? s_hi : s_lo;
+}
+template <int CTRL>
+__device__ __forceinline__ float halve_last(float lo, bool upper) {  // pair (lo, nothing)
+    const float s_lo = lo + dpp_f<CTRL, 0xF>(lo);
+    return upper ? 0.0f : s_lo;
+}
+__device__ __forceinline__ float wave_totals9(const float (&g)[NG2D], uint32_t lane) {
+    static_assert(NG2D == 9, "the halving steps are laid out for nine values");
+    const bool b3 = lane & 8u, b2 = lane & 4u, b1 = lane & 2u, b0 = lane & 1u;
+    float a[5];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = halve_pair<0x140>(g[j], g[j + 5], b3);
+    a[4] = halve_last<0x140>(g[4], b3);
+    float b[3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) b[k] = halve_pair<0x141>(a[k], a[k + 3], b2);
+    b[2] = halve_last<0x141>(a[2], b2);
+    const float c0 = halve_pair<0x4E>(b[0], b[2], b1), c1 = halve_last<0x4E>(b[1], b1);
+    float d = halve_pair<0xB1>(c0, c1, b0);
+    const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(d), __float_as_uint(d), false, false);
+    d = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);  // rows 0+1 and 2+3
+    const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(d), __float_as_uint(d), false, false);
+    return __uint_as_float(r32[0]) + __uint_as_float(r32[1]);  // all four rows
+}
+// Which of the nine values lane `lane` (< 16) holds after wave_totals9, or
+// NG2D when it holds none: the slot chosen at each halving, read back.
+__device__ __forceinline__ uint32_t totals9_index(uint32_t lane) {
+    const uint32_t cs = lane & 1u;                    // of c0, c1
+    const uint32_t bs = (lane & 2u) ? 2u + cs : cs;   // of b[0..2] (3: none)
+    const uint32_t as = (lane & 4u) ? 3u + bs : bs;   // of a[0..4]
+    const uint32_t vi = (lane & 8u) ? 5u + as : as;   // of g[0..8]
+    return (bs < 3u && as < 5u && vi < (uint32_t)NG2D) ? vi : (uint32_t)NG2D;
+}
+
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
     const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ powner,
@@ -70,6 +120,8 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const uint32_t px = qx0 + (lane_i & 7u), py = qy0 + (lane_i >> 3);
     const uint32_t lx = px - ox, ly = py - oy;
     const bool inside = px < v.w && py < v.h;
+    // The value this lane writes after each nine-value wave reduction.
+    const uint32_t sum_slot = lane_i < 16u ? totals9_index(lane_i) : (uint32_t)NG2D;
     const float fx = (float)ox;
     const float gx = (float)(lx & ~3u);
     const float lane = (float)(lx & 3u);
@@ -171,12 +223,9 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
                     gv[4] = wdy * dy;
                 }
             }
-            if (__any(con)) {  // wave-uniform, so all 64 lanes take part in the DPP sums
-#pragma unroll
-                for (int c = 0; c < NG2D; ++c) {
-                    const float sum = wave_total_f(gv[c]);
-                    if (lane_i == 63) s_part[w][c][k] = sum;
-                }
+            if (__any(con)) {  // wave-uniform, so all 64 lanes take part in the exchanges
+                const float sum = wave_totals9(gv, lane_i);
+                if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
             }
         }
         __syncthreads();
