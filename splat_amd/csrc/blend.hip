@@ -84,6 +84,8 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     // is never NaN, so this is the reference's stop at T < T_MIN); deriving it
     // from T keeps it a plain compare rather than a loop-carried mask.
     const auto live = [&]() { return inside && T >= T_MIN; };
+    const uint64_t inside_mask = lane_mask(inside);
+    const auto live_mask = [&]() { return lane_mask(T >= T_MIN) & inside_mask; };
     uint32_t last = 0;
     bool finished = false;
     const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
@@ -108,11 +110,12 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
         // Records of the batch that can reach this wave's quadrant, 64 at a
         // time: one LDS read per lane and a ballot, then the set bits in order.
-        bool open = __any(live());
+        bool open = live_mask() != 0ull;
         for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
             unsigned long long reach = __ballot(g0 + lane_i < nb && ((s_qm[g0 + lane_i] >> w) & 1u));
             while (reach) {
-                if (!__any(live())) {  // every pixel of the quadrant saturated
+                const uint64_t lm = live_mask();
+                if (lm == 0ull) {  // every pixel of the quadrant saturated
                     open = false;
                     break;
                 }
@@ -121,6 +124,8 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 const uint4 r2 = s_rec[k][2];
                 const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
                 const bool act = live() && px >= bx0 && px < bx1 && py >= by0 && py < by1;
+                const uint64_t act_mask =
+                    lm & lane_mask(px >= bx0) & lane_mask(px < bx1) & lane_mask(py >= by0) & lane_mask(py < by1);
                 const uint4 r0 = s_rec[k][0];
                 const uint4 r1 = s_rec[k][1];
                 const float cx0 = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
@@ -134,7 +139,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 const float pw = (qx + qy) + bdy * dx;
                 // Lanes below the splat's threshold have alpha < ALPHA_MIN (the
                 // reference masks them); a wave with none left skips the rest.
-                if (!__any(act && pw >= __uint_as_float(r2.w))) continue;
+                if ((act_mask & lane_mask(pw >= __uint_as_float(r2.w))) == 0ull) continue;
                 const float op = __uint_as_float(r1.y);
                 const float alpha = fminf(exp_neg_times(pw, op), ALPHA_MAX);
                 if (act && pw <= 0.0f && alpha >= ALPHA_MIN) {

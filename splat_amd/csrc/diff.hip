@@ -171,7 +171,9 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             const uint4 r2 = s_rec[k][2];
             const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
             const bool act = pos < last && px >= bx0 && px < bx1 && py >= by0 && py < by1;
-            if (!__any(act)) continue;
+            const uint64_t act_mask = lane_mask(pos < last) & lane_mask(px >= bx0) & lane_mask(px < bx1) &
+                                      lane_mask(py >= by0) & lane_mask(py < by1);
+            if (act_mask == 0ull) continue;
             const uint4 r0 = s_rec[k][0];
             const uint4 r1 = s_rec[k][1];
             const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
@@ -185,12 +187,14 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             const float qy = (nc * dy) * dy;
             const float bdy = nb2 * dy;
             const float pw = (qx + qy) + bdy * dx;
-            if (!__any(act && pw >= __uint_as_float(r2.w))) continue;
+            if ((act_mask & lane_mask(pw >= __uint_as_float(r2.w))) == 0ull) continue;
             const float op = __uint_as_float(r1.y);
             const float ex = exp_neg(pw);
             const float raw = ex * op;
             const float alpha = fminf(raw, ALPHA_MAX);
             const bool con = act && pw <= 0.0f && alpha >= ALPHA_MIN;
+            // Voted here, where every lane is active, so the masks need no merging.
+            const bool any_con = (act_mask & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN)) != 0ull;
             float gv[NG2D];
 #pragma unroll
             for (int c = 0; c < NG2D; ++c) gv[c] = 0.0f;
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
                     gv[4] = wdy * dy;
                 }
             }
-            if (__any(con)) {  // wave-uniform, so all 64 lanes take part in the exchanges
+            if (any_con) {  // wave-uniform, so all 64 lanes take part in the exchanges
                 const float sum = wave_totals9(gv, lane_i);
                 if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
             }

@@ -111,8 +111,9 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
                 // Sorted or clustered keys often give a whole wave one digit: one
                 // add then replaces 64 colliding LDS atomics (s_h[w] is this wave's own).
                 const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-                const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // all lanes vote
-                if (__all(!valid || d == d0)) {
+                const uint64_t vm = lane_mask(valid);  // all lanes vote
+                const uint32_t nvalid = (uint32_t)__popcll(vm);
+                if ((vm & ~lane_mask(d == d0)) == 0ull) {  // every key of the wave has digit d0
                     if (lane == 0 && nvalid) atomicAdd(&s_h[w][d0], nvalid);
                 } else if (valid) {
                     atomicAdd(&s_h[w][d], 1u);
@@ -178,15 +179,15 @@ __device__ __forceinline__ void downsweep_tile(const uint32_t *__restrict__ ks, 
         // one digit needs no bit tests; otherwise one ballot per bit that the
         // pass carries (digits are the key's bits [shift, shift + width): bits
         // at and above `bits` do not order the keys and may carry data).
-        unsigned long long m = __ballot(valid);
+        unsigned long long m = lane_mask(valid);
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (!__all(!valid || d == d0)) {
+        if ((m & ~lane_mask(d == d0)) != 0ull) {
             const int nd = (int)~d;
             for (int b = 0; b < width; ++b) {
                 // x = all ones where this lane's bit b is 0; bb = lanes whose bit b is 1;
                 // bb ^ x = lanes whose bit b matches this lane's.
                 const uint32_t x = (uint32_t)__builtin_amdgcn_sbfe(nd, b, 1);
-                const unsigned long long bb = __ballot(x == 0u);
+                const unsigned long long bb = lane_mask(x == 0u);
                 m &= bb ^ (((unsigned long long)x << 32) | x);
             }
         }

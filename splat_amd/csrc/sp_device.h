@@ -57,6 +57,15 @@ __device__ __forceinline__ uint32_t radix_columns(uint64_t n, uint32_t tile_keys
     return (t + g - 1) / g;
 }
 
+// Wave votes over the active lanes. The compiler takes a ballot of a single
+// compare straight from the compare's lane mask; a ballot of a combined
+// predicate (and HIP's __any / __all, which go through an int) costs a
+// v_cndmask and a second compare. Hot loops therefore ballot each compare on
+// its own (lane_mask) and combine the masks on the scalar unit.
+__device__ __forceinline__ uint64_t lane_mask(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+__device__ __forceinline__ bool wave_any(bool c) { return lane_mask(c) != 0ull; }
+__device__ __forceinline__ bool wave_all(bool c) { return lane_mask(!c) == 0ull; }
+
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, false);
