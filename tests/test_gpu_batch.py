@@ -27,6 +27,9 @@ def test_batch_matches_single_views_and_cpu(oracle):
         assert np.array_equal(imgs[k], one.pixels), f"view {k}: batch differs from single render"
         assert np.array_equal(imgs[k], ref), f"view {k}: batch differs from cpu"
         assert (stats[k].visible, stats[k].overlaps) == (rst.visible, rst.overlaps)
+        # One view on a handle sized for 12: its counts sit at the start of the
+        # per-view arrays, which the stats readback must still find.
+        assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps), f"view {k}: single-view stats"
 
 
 def test_graph_replays_match_eager(oracle):
