@@ -195,42 +195,40 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             const bool con = act && pw <= 0.0f && alpha >= ALPHA_MIN;
             // Voted here, where every lane is active, so the masks need no merging.
             const bool any_con = (act_mask & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN)) != 0ull;
+            if (!any_con) continue;  // no lane takes this record: no gradient, T and S unchanged
+            // Branch-free over the wave: a lane without a contribution gets
+            // zero weights (aT, gs), so its gradients are zero and its T and S
+            // stay as they are.
+            const float om = 1.0f - alpha;
+            // One hardware reciprocal (1 ulp) for both divisions: gradients
+            // need no bit-exactness, and T's recovery drifts by ulps only.
+            const float rom = __builtin_amdgcn_rcpf(om);
+            const float Ti = T * rom;
+            const float c0 = __uint_as_float(r1.z), c1 = __uint_as_float(r1.w), c2 = __uint_as_float(r2.x);
+            const float aT = con ? alpha * Ti : 0.0f;
             float gv[NG2D];
-#pragma unroll
-            for (int c = 0; c < NG2D; ++c) gv[c] = 0.0f;
-            if (con) {
-                const float om = 1.0f - alpha;
-                // One hardware reciprocal (1 ulp) for both divisions: gradients
-                // need no bit-exactness, and T's recovery drifts by ulps only.
-                const float rom = __builtin_amdgcn_rcpf(om);
-                const float Ti = T * rom;
-                const float c0 = __uint_as_float(r1.z), c1 = __uint_as_float(r1.w), c2 = __uint_as_float(r2.x);
-                const float aT = alpha * Ti;
-                gv[6] = G0 * aT;
-                gv[7] = G1 * aT;
-                gv[8] = G2 * aT;
-                const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) * rom;
-                S0 += c0 * aT;
-                S1 += c1 * aT;
-                S2 += c2 * aT;
-                T = Ti;
-                if (raw < ALPHA_MAX) {
-                    gv[5] = ga * ex;
-                    // Moments of dL/dpw over the pixels; the record's conic
-                    // coefficients are applied once per record after the sums.
-                    const float gpw = ga * alpha;
-                    const float u = gpw * dx, wdy = gpw * dy;
-                    gv[0] = u;
-                    gv[1] = wdy;
-                    gv[2] = u * dx;
-                    gv[3] = u * dy;
-                    gv[4] = wdy * dy;
-                }
-            }
-            if (any_con) {  // wave-uniform, so all 64 lanes take part in the exchanges
-                const float sum = wave_totals9(gv, lane_i);
-                if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
-            }
+            gv[6] = G0 * aT;
+            gv[7] = G1 * aT;
+            gv[8] = G2 * aT;
+            const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) * rom;
+            S0 += c0 * aT;
+            S1 += c1 * aT;
+            S2 += c2 * aT;
+            T = con ? Ti : T;
+            // Opacity and falloff terms vanish where alpha is capped.
+            const float gs = (con && raw < ALPHA_MAX) ? ga : 0.0f;
+            gv[5] = gs * ex;
+            // Moments of dL/dpw over the pixels; the record's conic
+            // coefficients are applied once per record after the sums.
+            const float gpw = gs * alpha;
+            const float u = gpw * dx, wdy = gpw * dy;
+            gv[0] = u;
+            gv[1] = wdy;
+            gv[2] = u * dx;
+            gv[3] = u * dy;
+            gv[4] = wdy * dy;
+            const float sum = wave_totals9(gv, lane_i);  // wave-uniform: all 64 lanes take part
+            if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
         }
         __syncthreads();
         if (threadIdx.x < nb) {
