@@ -1,8 +1,8 @@
 // Device-wide exclusive scans in reduce-then-scan form (no cross-block
 // waiting): producers write one count per block into a row; these kernels
 // scan the rows (one workgroup per row, all rows in parallel); consumers
-// re-read their block's offset. Used by the projection compaction, the pair
-// emission and the radix sort.
+// re-read their block's offset. Used by the pair emission, the binned tile
+// lists and the radix sort.
 #include <algorithm>
 
 #include "sp_device.h"
