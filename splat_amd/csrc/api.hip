@@ -182,7 +182,8 @@ struct sp_handle {
     // device memory, per-pair owners, per-pixel blend state, gradient scratch.
     bool diff = false;
     sp::DevParams prm{};
-    uint32_t *d_powner = nullptr;  // pair_cap: record index per emission index
+    uint32_t *d_powner = nullptr;  // pair_cap: record index per emission slot (differentiable handles)
+    uint32_t *d_pbase = nullptr;   // view_cap x n: first emission slot of each record (differentiable handles)
     float *d_pixT = nullptr;       // pix_cap: final transmittance per pixel (256 per tile)
     uint32_t *d_pixlast = nullptr; // pix_cap: tile-list position after the last contribution
     uint64_t pix_cap = 0;
@@ -308,6 +309,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     dfree(h, h->d_rec, old * nn * 48);
     dfree(h, h->d_box, old * nn * 8);
     dfree(h, h->d_sbox, old * nn * 8);
+    if (h->diff) dfree(h, h->d_pbase, old * nn * 4);
     for (int b = 0; b < 2; ++b) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
@@ -328,6 +330,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
     int rc = dalloc(h, (void **)&h->d_rec, v * nn * 48);
     if (!rc) rc = dalloc(h, (void **)&h->d_box, v * nn * 8);
     if (!rc) rc = dalloc(h, (void **)&h->d_sbox, v * nn * 8);
+    if (!rc && h->diff) rc = dalloc(h, (void **)&h->d_pbase, v * nn * 4);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
@@ -490,8 +493,8 @@ int setup(sp_handle *h) {
 // Enqueue the whole pipeline for the first `nviews` entries of h_views. The
 // tile lists of the (last) phase end in ping-pong buffer *pair_buf. The first
 // phase is binned (superblock pairs sorted, tile lists written per
-// superblock) except on the differentiable path, whose pairs carry emission
-// indices and go through the tile sort.
+// superblock); the differentiable path also numbers its tile pairs in
+// emission order (powner, pbase) for the adjoint.
 // BL_RGB with a split runs the emission, tile sort, ranges and blend twice:
 // the first phase for each view's nearest splats over all tiles, the second
 // for the rest over the tiles the first blend left unfinished.
@@ -520,15 +523,13 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     h->depth_buf = di;
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
     uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
-    const bool binned = mode != BL_DIFF;
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
                             es, nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs,
-                            binned ? h->d_bsb : nullptr,
-                            binned ? vsb : nullptr, h->d_pkey[0], h->d_pval[0], powner, h->pair_cap,
-                            h->d_counters));
+                            h->d_bsb, vsb, h->d_pkey[0], h->d_pval[0], powner, powner ? h->d_pbase : nullptr,
+                            h->pair_cap, h->d_counters));
     HIP_TRY(mark(1));
     int pi = 0;
-    if (binned) {
+    {
         int si = 0;
         const int sb_bits = (int)bits_for(total_sbs ? total_sbs - 1 : 0);
         HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_SBPAIRS, 1, 0, h->pair_cap,
@@ -542,16 +543,11 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
                                      h->d_tile_start,
                                      mode == BL_NONE ? h->d_pkey[pi] : nullptr, h->d_pval[pi], h->pair_cap,
                                      h->d_counters));
-    } else {
-        HIP_TRY(sp::launch_radix_sort(s, h->d_pkey, h->d_pval, h->d_counters + sp::CNT_PAIRS, 1, 0, h->pair_cap,
-                                      tile_bits, h->d_rowtot, h->d_table, &pi));
-        HIP_TRY(sp::launch_ranges(s, h->d_pkey[pi], 0xFFFFFFFFu, h->d_counters + sp::CNT_PAIRS, h->pair_cap,
-                                  total_tiles, h->d_tile_start));
     }
     HIP_TRY(mark(2));
     if (mode != BL_NONE) {
         const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
-        HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi], powner,
+        HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
                                  h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast, h->d_done, h->d_pstate,
                                  h->blend_batch));
     }
@@ -560,7 +556,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip, vopen));
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
                                 h->d_views, es, h->d_ip, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
-                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
+                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));
     if (split) {
@@ -572,7 +568,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(mark(5));
     if (split)
         HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
-                                 nullptr, h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
+                                 h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
                                  h->blend_batch));
     HIP_TRY(mark(6));
     // Counters, visible counts and pair counts (view_cap apart) in one copy.
@@ -611,9 +607,9 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     const uint32_t total_tiles = (uint32_t)total;
     rc = ensure_tiles(h, total_tiles);
     if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
-    if (!rc && mode != BL_DIFF && total_sb >= (1ull << sp::SB_SUB_SHIFT))
+    if (!rc && total_sb >= (1ull << sp::SB_SUB_SHIFT))
         return fail(SP_E_INVALID, "too many superblocks in one batch");
-    if (!rc && mode != BL_DIFF) rc = ensure_bin(h, total_tiles, total_sb);
+    if (!rc) rc = ensure_bin(h, total_tiles, total_sb);
     if (!rc && mode == BL_RGB && h->split_shift > 0) rc = ensure_split(h, total_tiles);
     if (rc) return rc;
     const int tile_bits = (int)bits_for(total_tiles ? total_tiles - 1 : 0);
@@ -662,10 +658,8 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         // Pair counts saturate at 2^32 - 1 (the scans never wrap): more pairs
         // than a u32 index addresses cannot be rendered (the reference's CPU
         // backend stops there too, src/render/cpu.rs:158-160).
-        const bool binned_first = mode != BL_DIFF;
         const bool split_on = mode == BL_RGB && h->split_shift > 0;
-        if (h->h_counters[sp::CNT_PAIRS] == 0xFFFFFFFFu ||
-            (binned_first && h->h_counters[sp::CNT_SBPAIRS] == 0xFFFFFFFFu) ||
+        if (h->h_counters[sp::CNT_PAIRS] == 0xFFFFFFFFu || h->h_counters[sp::CNT_SBPAIRS] == 0xFFFFFFFFu ||
             (split_on && h->h_counters[sp::CNT_PAIRS2] == 0xFFFFFFFFu))
             return fail(SP_E_CAPACITY, "more than 2^32 (gaussian, tile) pairs in one phase of the frame");
         if (h->h_counters[sp::CNT_OVERFLOW] && attempt < 3) {
@@ -850,6 +844,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_rgb);
     (void)hipFree(h->d_rowtot);
     (void)hipFree(h->d_powner);
+    (void)hipFree(h->d_pbase);
     (void)hipFree(h->d_pixT);
     (void)hipFree(h->d_pixlast);
     (void)hipFree(h->d_gpair);
@@ -1312,7 +1307,7 @@ int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *
     }
     HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, (uint64_t)nv * h->n * sp::NG2D, 0.0f));  // the records in use
     HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->d_tile_start, h->d_pval[h->pair_buf],
-                                 h->d_powner, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
+                                 h->d_pbase, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
                                  h->d_stamp, h->stamp_id));
     HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_stamp, h->stamp_id, h->d_powner, np, h->d_g2d,
                                   h->d_part));

@@ -22,8 +22,8 @@
 //
 // Modes:
 //   BM_RGB     one list per tile, RGB8 output.
-//   BM_DIFF    the differentiable forward: pairs carry emission indices (the
-//              record index is powner[e]); the output is the float image
+//   BM_DIFF    the differentiable forward (lists of record indices, like
+//              the others): the output is the float image
 //              C + T*bg, and each pixel's final transmittance and the
 //              tile-list position after its last contribution are kept for
 //              the adjoint (pix_T / pix_last, 256 per tile in thread order).
@@ -44,7 +44,6 @@ template <int BATCH, int MODE>
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
                                                          uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
-                                                         const uint32_t *__restrict__ powner,
                                                          const uint4 *__restrict__ rec, uint32_t per_xcd,
                                                          uint32_t *__restrict__ counters, float *__restrict__ pix_T,
                                                          uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
@@ -94,7 +93,6 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         const uint32_t idx = b + threadIdx.x;
         if (threadIdx.x < BATCH && idx < end) {
             uint32_t id = sval[idx];
-            if (MODE == BM_DIFF) id = powner[id];
             const uint4 *r = rec + 3 * (uint64_t)id;
             uint4 r0 = r[0];
             const uint4 r1 = r[1], r2 = r[2];
@@ -188,22 +186,22 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
 
 template <int MODE>
 void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                 const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                 const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                  uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate, int batch) {
     const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
     const dim3 grid(per_xcd * XCDS), block(BLEND_THREADS);
     switch (batch) {
     case 64:
         hipLaunchKernelGGL((k_blend<64, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     case 128:
         hipLaunchKernelGGL((k_blend<128, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     default:
         hipLaunchKernelGGL((k_blend<256, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           powner, rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
         break;
     }
 }
@@ -211,26 +209,26 @@ void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32
 }  // namespace
 
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch) {
     if (total_tiles == 0) return hipSuccess;
     float4 *ps = reinterpret_cast<float4 *>(pstate);
     switch (mode) {
     case BM_DIFF:
-        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T, pix_last,
+        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T, pix_last,
                              done, ps, batch);
         break;
     case BM_FIRST:
-        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T,
+        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T,
                               pix_last, done, ps, batch);
         break;
     case BM_SECOND:
-        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T,
+        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T,
                                pix_last, done, ps, batch);
         break;
     default:
-        launch_mode<BM_RGB>(s, views, nviews, total_tiles, tile_start, sval, powner, rec, counters, pix_T, pix_last,
+        launch_mode<BM_RGB>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T, pix_last,
                             done, ps, batch);
         break;
     }

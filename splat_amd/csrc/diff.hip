@@ -1,11 +1,13 @@
 // Differentiation: the adjoint of the blend and of the projection.
 //
 // The differentiable forward renders float images (C + T*bg) with the same
-// pipeline as the plain path, except that the tile sort carries each pair's
-// emission index e (pairs are emitted splat by splat in depth order, so the
-// pairs of one (view, splat) record are contiguous in e) and the blend keeps,
-// per pixel, its final transmittance and the tile-list position after its
-// last contribution. The backward is three passes, none of which uses a
+// binned pipeline as the plain path (no front-to-back split). Its emission
+// also numbers the tile pairs in emission order, view by view and splat by
+// splat in depth order, so the pairs of one (view, splat) record are
+// contiguous: the record of each slot (powner) and each record's first slot
+// (pbase); a pair's slot e is pbase[record] plus the tile's row-major place
+// in the record's tile rectangle. The blend keeps, per pixel, its final
+// transmittance and the tile-list position after its last contribution. The backward is three passes, none of which uses a
 // floating-point atomic, so gradients are deterministic:
 //
 //   1. k_blend_bwd: one workgroup per tile walks the tile's list back to
@@ -22,7 +24,7 @@
 //      wave, then the four waves);
 //      the mean and conic terms are summed as moments of dL/dpw (dx, dy,
 //      dx^2, dx dy, dy^2) and multiplied by the conic once per record. They are
-//      stored per pair at its emission index. Records whose falloff cannot
+//      stored per pair at its emission slot. Records whose falloff cannot
 //      reach a quadrant are never visited by its wave (one ballot over the
 //      staged records' reach bits).
 //   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
@@ -98,7 +100,7 @@ __device__ __forceinline__ uint32_t totals9_index(uint32_t lane) {
 
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
-    const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ powner,
+    const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ pbase,
     const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
     float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ stamp, uint32_t stamp_id,
     uint32_t per_xcd) {
@@ -149,10 +151,15 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         const uint32_t lo = hi > (uint32_t)BWD_BATCH ? hi - BWD_BATCH : 0u;
         const uint32_t nb = hi - lo;
         if (threadIdx.x < nb) {
-            const uint32_t e = sval[start + lo + threadIdx.x];
-            s_e[threadIdx.x] = e;
-            const uint4 *r = rec + 3 * (uint64_t)powner[e];
+            const uint32_t id = sval[start + lo + threadIdx.x];  // record index
+            const uint4 *r = rec + 3 * (uint64_t)id;
             const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
+            // The pair's emission slot: the record's first slot plus this
+            // tile's row-major place in the record's tile rectangle, as the
+            // emission numbered them (tile rectangle from the pixel box).
+            const uint32_t tx0 = (r2.y & 0xFFFFu) / TILE, ty0 = (r2.y >> 16) / TILE;
+            const uint32_t rw = ((r2.z & 0xFFFFu) + TILE - 1u) / TILE - tx0;
+            s_e[threadIdx.x] = pbase[id] + (ty - ty0) * rw + (tx - tx0);
             s_rec[threadIdx.x][0] = r0;
             s_rec[threadIdx.x][1] = r1;
             s_rec[threadIdx.x][2] = r2;
@@ -610,13 +617,13 @@ void launch_pbwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uin
 }  // namespace
 
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
+                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
     hipLaunchKernelGGL(k_blend_bwd, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, views, nviews, total_tiles,
-                       tile_start, sval, powner, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id, per_xcd);
+                       tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id, per_xcd);
     return hipGetLastError();
 }
 

@@ -70,19 +70,22 @@ uint32_t emit_blocks(uint32_t n);
 // bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
 // entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
 // (SECOND), saturated at 0xFFFFFFFF, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
-// also writes each view's unsplit pair count to view_pairs[v]. With powner
-// set, pval[e] = e and powner[e] = record index instead.
+// also writes each view's unsplit pair count to view_pairs[v].
 // box: n entries per view, gathered by FIRST from the scene-order boxes sbox (the
 // depth-ordered pixel boxes) and read by SECOND.
 // Binned FIRST (bsb / vsb given, shaped like bcnt / vtot): the pairs are
 // (global superblock id | sub-rectangle << SB_SUB_SHIFT, record index)
 // instead, counted in counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts
-// the tile pairs.
+// the tile pairs. With powner (binned FIRST only), also powner[e] = the
+// record of emission slot e (tile pairs numbered view by view, splat by splat
+// in depth order, each splat's rectangle row-major) and pbase[record] = the
+// record's first slot.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
-                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters);
+                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
+                       uint32_t *counters);
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
 // entries, trow bin_scan_rows.
 uint32_t bin_scan_rows(uint32_t total_tiles);
@@ -107,23 +110,24 @@ hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask,
 
 // ---- blend.hip: (e) front-to-back blend of every tile of every view ----
 // mode BM_RGB / BM_FIRST / BM_SECOND write RGB8 (ViewParams::out); BM_DIFF
-// takes emission-index pairs (record = powner[e]) and writes the float image
-// (ViewParams::fimg) and the per-pixel final transmittance / last
+// writes the float image (ViewParams::fimg) and the per-pixel final transmittance / last
 // contributing list position (pix_T / pix_last, total_tiles * 256 each).
 // BM_FIRST sets done[t] per tile and keeps unfinished tiles' pixel state in
 // pstate (total_tiles * 256 * 4 floats); BM_SECOND resumes from it.
 // batch = records staged through LDS per step (64, 128 or 256).
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner, const uint4 *rec,
+                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch);
 
 // ---- diff.hip: adjoint of the blend and of the projection ----
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission
-// index) of every tile's list up to its last used position, each written pair
-// stamped with stamp_id; pairs after it are left untouched.
+// slot: pbase[record] + the tile's row-major place in the record's tile
+// rectangle) of every tile's list (record indices, sval) up to its last used
+// position, each written pair stamped with stamp_id; pairs after it are left
+// untouched.
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *powner,
+                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id);
 // Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.

@@ -14,12 +14,13 @@
 // keeps one workgroup per splat block. Pairs come out view by view in
 // (depth, scene index) order, carrying a global tile id (view tile base +
 // tile) and a record index (view * n + scene index), ready for the stable
-// tile-id sort. The differentiable path sorts the emission index instead and
-// keeps the record index per emission index (powner), so that per-pair
-// gradients can be written back in emission order, where each splat's pairs
-// are contiguous. The plain path's first phase is binned instead of sorted
-// (superblock pairs, then per-superblock tile lists; see "Binned first
-// phase" below).
+// tile-id sort. The first phase is binned instead of sorted (superblock
+// pairs, then per-superblock tile lists; see "Binned first phase" below).
+// The differentiable path also numbers the tile pairs in emission order: the
+// record of each emission slot (powner) and each record's first slot
+// (pbase), so that per-pair gradients can be written back in emission order,
+// where each splat's pairs are contiguous (a pair's slot is its record's
+// first slot plus the tile's row-major place in the record's rectangle).
 //
 // Front-to-back split. Almost every tile saturates (all pixels reach
 // T < 1e-4) within the first few percent of its depth-ordered list, so the
@@ -231,8 +232,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
                                                               const uint32_t *__restrict__ boff,
                                                               const uint32_t *__restrict__ vbase_pairs,
                                                               uint32_t *__restrict__ pkey, uint32_t *__restrict__ pval,
-                                                              uint32_t *__restrict__ powner, uint64_t pair_cap,
-                                                              uint32_t *__restrict__ counters) {
+                                                              uint64_t pair_cap, uint32_t *__restrict__ counters) {
     __shared__ uint32_t s_pre[EMIT_THREADS];
     __shared__ uint32_t s_org[EMIT_THREADS];  // first global tile id of the rectangle
     __shared__ uint32_t s_rw[EMIT_THREADS];   // rectangle width in tiles
@@ -330,14 +330,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
                     continue;
                 }
                 pkey[qq] = tile;
-                if (powner) {
-                    // Differentiable path: the pair carries its emission index; the
-                    // record index stays behind in emission order.
-                    pval[qq] = (uint32_t)qq;
-                    powner[qq] = s_id[lo];
-                } else {
-                    pval[qq] = s_id[lo];
-                }
+                pval[qq] = s_id[lo];
             }
             q = hi;
             // Search again at a view's end (its last blocks may be empty) or past its last block.
@@ -723,13 +716,59 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
     }
 }
 
+// Differentiable path: the emission numbering of the first phase's tile
+// pairs (view by view, splat by splat in depth order, each rectangle
+// row-major), one thread per depth-ordered splat: pbase[record] = the
+// record's first slot and powner[e] = the record of slot e. The slots come
+// from the same block offsets (boff) and view bases (vbase_pairs) as the
+// emission's; slots at or past pair_cap are not written (the frame is then
+// rendered again with more room).
+__global__ __launch_bounds__(EMIT_THREADS) void k_emit_owners(const uint32_t *__restrict__ dval, uint32_t n_splats,
+                                                        uint32_t bpv, const uint32_t *__restrict__ vis_count,
+                                                        const uint2 *__restrict__ box,
+                                                        const ViewParams *__restrict__ views, EmitSplit split,
+                                                        const uint32_t *__restrict__ boff,
+                                                        const uint32_t *__restrict__ vbase_pairs,
+                                                        uint32_t *__restrict__ powner, uint32_t *__restrict__ pbase,
+                                                        uint64_t pair_cap) {
+    const uint32_t v = blockIdx.x / bpv, blk = blockIdx.x % bpv;
+    const uint64_t vbase = (uint64_t)v * n_splats;
+    const uint32_t nvis = min(vis_count[v], n_splats);
+    if (blk * EMIT_THREADS >= nvis) return;  // block-uniform, before the scan's barriers
+    const uint32_t j = blk * EMIT_THREADS + threadIdx.x;
+    const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
+    const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
+    const uint32_t c = phase_count<EMIT_FIRST>(r, j, first_count(nvis, split), nullptr, tile_base, tiles_x);
+    uint32_t total;
+    const uint32_t pre = block_exclusive_scan<EMIT_THREADS>(c, &total);
+    // Slots are u32 (list entries are); clamped to the capacity.
+    const uint64_t lim = min(pair_cap, (uint64_t)0xFFFFFFFFu);
+    const uint64_t base = (uint64_t)vbase_pairs[v] + boff[blockIdx.x] + pre;
+    const uint32_t b = (uint32_t)min(base, lim), e = (uint32_t)min(base + c, lim);
+    if (c > 0) pbase[r.id] = (uint32_t)base;
+    // A short run is filled by its own lane; a long one (a splat over many
+    // tiles) by the whole wave, 64 consecutive slots per store.
+    const bool longrun = e - b > 16u;
+    if (!longrun)
+        for (uint32_t q = b; q < e; ++q) powner[q] = r.id;
+    uint64_t m = lane_mask(longrun);  // every lane of the wave is active here
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const uint32_t lb = __builtin_amdgcn_readlane(b, l), le = __builtin_amdgcn_readlane(e, l);
+        const uint32_t lid = __builtin_amdgcn_readlane(r.id, l);
+        for (uint32_t q = lb + lane_id(); q < le; q += 64u) powner[q] = lid;
+    }
+}
+
 template <int PHASE>
 hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                       EmitSplit split,
                       const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *bsb, uint32_t *vsb, uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
-                      uint32_t *powner, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
+                      uint32_t *powner, uint32_t *pbase, uint64_t pair_cap, uint32_t count_slot,
+                      uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     if (PHASE == EMIT_FIRST) {
         hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
@@ -754,7 +793,13 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
         const uint64_t chunks = (pair_cap + EXPAND_CHUNK_BIN_MIN - 1) / EXPAND_CHUNK_BIN_MIN;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_dense<true>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, nullptr, pair_cap, counters);
+                           nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, pair_cap, counters);
+        if (powner) {  // the differentiable path also numbers the tile pairs in emission order
+            const hipError_t eb = hipGetLastError();
+            if (eb != hipSuccess) return eb;
+            hipLaunchKernelGGL(k_emit_owners, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+                               vis_count, box, views, split, bcnt, vtot, powner, pbase, pair_cap);
+        }
     } else if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            vis_count, box, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
@@ -762,7 +807,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
         const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_dense<false>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           nviews, vis_count, box, views, split, bcnt, vtot, pkey, pval, powner, pair_cap, counters);
+                           nviews, vis_count, box, views, split, bcnt, vtot, pkey, pval, pair_cap, counters);
     }
     return hipGetLastError();
 }
@@ -775,13 +820,15 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
-                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint64_t pair_cap, uint32_t *counters) {
+                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
+                       uint32_t *counters) {
     if (phase == EMIT_SECOND)
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt,
-                                       bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, powner,
-                                       pair_cap, CNT_PAIRS2, counters);
+                                       bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, nullptr,
+                                       nullptr, pair_cap, CNT_PAIRS2, counters);
+    if (powner && !bsb) return hipErrorInvalidValue;  // emission numbering comes with the binned lists
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
-                                  vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pair_cap, CNT_PAIRS,
+                                  vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pbase, pair_cap, CNT_PAIRS,
                                   counters);
 }
 
