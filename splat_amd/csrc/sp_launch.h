@@ -73,10 +73,10 @@ uint32_t emit_blocks(uint32_t n);
 // also writes each view's unsplit pair count to view_pairs[v].
 // box: n entries per view, gathered by FIRST from the scene-order boxes sbox (the
 // depth-ordered pixel boxes) and read by SECOND.
-// Binned FIRST (bsb / vsb given, shaped like bcnt / vtot): the pairs are
+// FIRST is binned (bsb / vsb required, shaped like bcnt / vtot): its pairs are
 // (global superblock id | sub-rectangle << SB_SUB_SHIFT, record index)
 // instead, counted in counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts
-// the tile pairs. With powner (binned FIRST only), also powner[e] = the
+// the tile pairs. With powner (FIRST only), also powner[e] = the
 // record of emission slot e (tile pairs numbered view by view, splat by splat
 // in depth order, each splat's rectangle row-major) and pbase[record] = the
 // record's first slot.

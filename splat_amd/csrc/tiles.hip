@@ -803,11 +803,6 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     } else if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            vis_count, box, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
-    } else {
-        const uint64_t chunks = (pair_cap + EXPAND_CHUNK - 1) / EXPAND_CHUNK;
-        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
-        hipLaunchKernelGGL(k_emit_expand_dense<false>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           nviews, vis_count, box, views, split, bcnt, vtot, pkey, pval, pair_cap, counters);
     }
     return hipGetLastError();
 }
@@ -826,7 +821,7 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt,
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, nullptr,
                                        nullptr, pair_cap, CNT_PAIRS2, counters);
-    if (powner && !bsb) return hipErrorInvalidValue;  // emission numbering comes with the binned lists
+    if (!bsb || !vsb) return hipErrorInvalidValue;  // the first phase is always binned
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
                                   vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pbase, pair_cap, CNT_PAIRS,
                                   counters);
