@@ -179,6 +179,15 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
                    uint64_t *keys, uint32_t *vals, uint32_t *tile_start, uint64_t cap,
                    uint64_t *n_pairs);
 
+/* Reach bits of the blend for one projected splat: for every 16x16 tile of a
+ * width x height frame (row-major), bit q is set unless the splat provably
+ * blends no pixel of the tile's 8x8 quadrant (q & 1, q >> 1), i.e. unless
+ * alpha < ALPHA_MIN there under raster_tile's f32 arithmetic
+ * (src/render/cpu.rs:655-686). `proj` and `bbox` are one splat's
+ * sp_debug_project fields. Host code only (the same source the blend runs on
+ * the device); needs no device. */
+int sp_debug_reach(const float *proj, const uint16_t *bbox, uint32_t width, uint32_t height, uint8_t *masks);
+
 /* Work counters of the handle's last rendered view (no reference counterpart;
  * the roofline numerators of the benchmark). */
 typedef struct sp_frame_info {

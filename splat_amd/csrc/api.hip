@@ -28,6 +28,7 @@
 #include "renderer.hpp"
 #include "sp_common.h"
 #include "sp_launch.h"
+#include "sp_reach.h"
 #include "splat_hip.h"
 
 using sp::ViewCtx;
@@ -1047,6 +1048,29 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     HIP_TRY(hipMemcpy(dv.data(), h->d_dval[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
     for (uint32_t q = 0; q < nv; ++q) depth[dv[q]] = dk[q];
     for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | depth[vals[p]];
+    return SP_OK;
+}
+
+int sp_debug_reach(const float *proj, const uint16_t *bbox, uint32_t width, uint32_t height, uint8_t *masks) {
+    if (!proj || !bbox || !masks) return fail(SP_E_INVALID, "sp_debug_reach: NULL argument");
+    int rc = check_frame(width, height);
+    if (rc) return rc;
+    // The record as the projection packs it (sp_splat.h): x, y, -a/2, -b;
+    // -c/2, opacity, r, g; b, box lo, box hi, falloff threshold.
+    auto u = [](float f) {
+        uint32_t r;
+        std::memcpy(&r, &f, 4);
+        return r;
+    };
+    const float k = k_of(proj[8]);
+    const uint32_t w0[4] = {u(proj[0]), u(proj[1]), u(-0.5f * proj[2]), u(-proj[3])};
+    const uint32_t w1[4] = {u(-0.5f * proj[4]), u(proj[8]), u(proj[5]), u(proj[6])};
+    const uint32_t w2[4] = {u(proj[7]), (uint32_t)bbox[0] | ((uint32_t)bbox[1] << 16),
+                            (uint32_t)bbox[2] | ((uint32_t)bbox[3] << 16), u(-0.5f * (k * k) - 0.02f)};
+    const uint32_t tx = (width + 15) / 16, ty = (height + 15) / 16;
+    for (uint32_t y = 0; y < ty; ++y)
+        for (uint32_t x = 0; x < tx; ++x)
+            masks[(uint64_t)y * tx + x] = (uint8_t)sp::quadrant_mask_words(w0, w1, w2, 16 * x, 16 * y);
     return SP_OK;
 }
 

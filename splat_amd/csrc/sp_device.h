@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sp_common.h"
+#include "sp_reach.h"
 
 namespace sp {
 
@@ -135,55 +136,12 @@ __device__ __forceinline__ float wave_total_f(float v) {
     return v;
 }
 
-// Upper bound of the falloff power over a rectangle of offsets
-// [dxa, dxb] x [dya, dyb] from the splat centre: the power
-// na*dx^2 + nc*dy^2 + nb2*dx*dy (na, nc < 0) is concave with its maximum 0 at
-// the centre; off the rectangle the maximum lies on an edge, where it is the
-// clamped vertex of a 1D parabola.
-__device__ __forceinline__ float max_power(float na, float nb2, float nc, float dxa, float dxb, float dya,
-                                           float dyb) {
-    if (dxa <= 0.0f && dxb >= 0.0f && dya <= 0.0f && dyb >= 0.0f) return 0.0f;
-    float m = -INFINITY;
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const float c = e ? dxb : dxa;  // edge dx = c
-        const float d = fminf(fmaxf(-nb2 * c / (2.0f * nc), dya), dyb);
-        m = fmaxf(m, (na * c * c + nc * d * d) + nb2 * c * d);
-    }
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const float c = e ? dyb : dya;  // edge dy = c
-        const float d = fminf(fmaxf(-nb2 * c / (2.0f * na), dxa), dxb);
-        m = fmaxf(m, (na * d * d + nc * c * c) + nb2 * c * d);
-    }
-    return m;
-}
-
-// Bit q (quadrant (q & 1, q >> 1) of the tile at ox, oy) is set when the
-// record's pixel box overlaps the quadrant and its power can reach the
-// record's threshold at some pixel centre of the overlap. A clear bit proves
-// that no pixel of the quadrant blends the record: the bound is taken over
-// the continuous rectangle of the in-box pixel centres and compared with a
-// margin of 0.01 (the threshold itself carries a 0.02 margin), so f32 rounding
-// cannot clear a bit the reference's arithmetic would need.
+// Reach bits of a staged record over the tile's four quadrants (sp_reach.h).
 __device__ __forceinline__ uint32_t quadrant_mask(const uint4 &r0, const uint4 &r1, const uint4 &r2, uint32_t ox,
                                                   uint32_t oy) {
-    const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-    const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
-    const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w), nc = __uint_as_float(r1.x);
-    const float thr = __uint_as_float(r2.w) - 0.01f;
-    uint32_t m = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint32_t qx0 = ox + 8u * (q & 1), qy0 = oy + 8u * (q >> 1);
-        const uint32_t xa = max(qx0, bx0), xb = min(qx0 + 8u, bx1);
-        const uint32_t ya = max(qy0, by0), yb = min(qy0 + 8u, by1);
-        if (xa >= xb || ya >= yb) continue;
-        const float pm = max_power(na, nb2, nc, ((float)xa + 0.5f) - sx, ((float)xb - 0.5f) - sx,
-                                   ((float)ya + 0.5f) - sy, ((float)yb - 0.5f) - sy);
-        if (!(pm < thr)) m |= 1u << q;  // NaN keeps the bit
-    }
-    return m;
+    const uint32_t w0[4] = {r0.x, r0.y, r0.z, r0.w}, w1[4] = {r1.x, r1.y, r1.z, r1.w};
+    const uint32_t w2[4] = {r2.x, r2.y, r2.z, r2.w};
+    return quadrant_mask_words(w0, w1, w2, ox, oy);
 }
 
 // Rust `as u32` on f32 (saturating, NaN -> 0).

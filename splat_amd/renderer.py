@@ -224,6 +224,19 @@ class HipRenderer:
         return keys[:k], vals[:k], ts
 
 
+def debug_reach(proj, bbox, width: int, height: int) -> np.ndarray:
+    """The blend's quadrant reach bits of one projected splat (a row of
+    debug_project's proj and bbox) for every 16x16 tile of a width x height
+    frame: u8[tiles_y, tiles_x], bit q for the quadrant (q & 1, q >> 1).
+    Runs the blend's own bound on the host; needs no device."""
+    tx, ty = (width + 15) // 16, (height + 15) // 16
+    out = np.zeros((ty, tx), np.uint8)
+    p = np.ascontiguousarray(proj, np.float32).reshape(12)
+    b = np.ascontiguousarray(bbox, np.uint16).reshape(4)
+    N.check(N.hip().sp_debug_reach(N.fptr(p), b.ctypes.data_as(N.u16p), width, height, out.ctypes.data_as(N.u8p)))
+    return out
+
+
 def create(backend: Backend, scene, **kw):
     """render::create (src/render/mod.rs:115-121)."""
     if backend == Backend.HIP:
