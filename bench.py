@@ -265,9 +265,15 @@ def main() -> None:
     print(json.dumps(result))
 
 
+def host_cpus_visible() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def host_cores() -> int:
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    return min(cores, 16)
+    """Threads of the host-side legs: every CPU this process may run on, up to the
+    host's share per GPU (16 on an MI355X box, also its OMP_NUM_THREADS)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(host_cpus_visible(), share))
 
 
 def bench_diff(S, torch, args) -> dict:
@@ -568,6 +574,7 @@ def cpu_baseline(S, scene, views, W, H, r, budget_s: float) -> dict:
         "value": round(n / el, 4),
         "unit": "views/s",
         "cores": cores,
+        "host_cpus_visible": host_cpus_visible(),
         "kind": "port",
         "sample": f"first {n} views of the 40-frame orbit, same scene and size ({el:.1f} s)",
         "hip_image_bit_exact_vs_cpu": exact,

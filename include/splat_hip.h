@@ -179,6 +179,20 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
                    uint64_t *keys, uint32_t *vals, uint32_t *tile_start, uint64_t cap,
                    uint64_t *n_pairs);
 
+/* Both tile lists of a real front-to-back split render of `view` (the
+ * handle's tuning): first_vals / first_start[0..tiles] = the first phase's
+ * lists (scene indices in blend order) over every tile, done[t] = 1 when the
+ * first blend finished tile t, second_vals / second_start = the continuation
+ * lists of the tiles not done. A tile's first list followed by its second
+ * list is CpuTiled's bucket of that tile after sort_hi32 + fix_ties
+ * (src/render/cpu.rs:149-214). With the split off (split_shift 0) the first
+ * lists are the whole lists and the second ones are empty. `cap` is the
+ * length of each value buffer; *n_first / *n_second receive the totals even
+ * when one exceeds cap (then SP_E_CAPACITY). */
+int sp_debug_split_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, uint32_t *first_vals,
+                         uint32_t *first_start, uint32_t *done, uint32_t *second_vals, uint32_t *second_start,
+                         uint64_t cap, uint64_t *n_first, uint64_t *n_second);
+
 /* Reach bits of the blend for one projected splat: for every 16x16 tile of a
  * width x height frame (row-major), bit q is set unless the splat provably
  * blends no pixel of the tile's 8x8 quadrant (q & 1, q >> 1), i.e. unless

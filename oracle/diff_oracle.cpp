@@ -497,6 +497,8 @@ double region_loss(const or_diff *D, const or_diff::PerView &V, uint32_t si, con
 
 }  // namespace
 
+extern "C" int or_resolve_threads(int threads);  // splat_oracle.cpp
+
 extern "C" {
 
 // Build the FP64 session: params as float arrays in the sp_params layout
@@ -512,7 +514,7 @@ or_diff *or_diff_create(const float *means, const float *log_scales, const float
     D->L.rest = (deg + 1) * (deg + 1) - 1;
     D->L.np = 14 + 3 * D->L.rest;
     D->n = n;
-    D->threads = threads > 0 ? threads : std::max(1u, std::thread::hardware_concurrency());
+    D->threads = or_resolve_threads(threads);
     for (int c = 0; c < 3; ++c) D->bg[c] = bg[c];
     const uint32_t np = D->L.np;
     D->P.assign(n * np, 0.0);

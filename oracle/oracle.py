@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         L.or_cpu_pairs.restype = C.c_int
         L.or_cpu_pairs.argtypes = [C.c_void_p, vw, C.c_uint32, C.c_uint32, N.u64p, N.u32p, N.u32p, C.c_uint64,
                                    N.u64p, C.c_int]
+        L.or_cpu_tile_stats.restype = C.c_int
+        L.or_cpu_tile_stats.argtypes = [C.c_void_p, vw, C.c_uint32, C.c_uint32, N.u32p, N.u32p, C.c_int]
         L.or_naive_render.restype = C.c_int
         L.or_naive_render.argtypes = [sc, vw, C.c_uint32, C.c_uint32, N.f32p, N.u8p, st]
         L.or_project.restype = None
@@ -117,6 +119,17 @@ class CpuTiled:
                 raise RuntimeError("pair overflow")
             k = int(n.value)
             return keys[:k], vals[:k], ts
+
+    def tile_stats(self, view, width, height):
+        """Per tile: list length and the entries raster_tile takes before every
+        pixel saturates (the length when the tile never saturates)."""
+        tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        lens = np.zeros(tiles, np.uint32)
+        done = np.zeros(tiles, np.uint32)
+        if lib().or_cpu_tile_stats(self._h, C.byref(view), width, height, lens.ctypes.data_as(N.u32p),
+                                   done.ctypes.data_as(N.u32p), self.threads):
+            raise RuntimeError("more than 2^32 gaussian/tile pairs")
+        return lens, done
 
     def orig(self) -> np.ndarray:
         o = np.zeros(len(self._scene), np.uint32)

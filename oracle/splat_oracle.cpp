@@ -34,6 +34,8 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <sched.h>
+#include <cstdlib>
 #include <vector>
 
 #include "splat_hip.h"
@@ -366,10 +368,19 @@ void parallel_for(size_t n, int threads, F &&fn) {
     for (auto &th : pool) th.join();
 }
 
+// Default thread count: the CPUs this process may run on, capped by
+// OMP_NUM_THREADS when it is set (a host's share of CPUs per GPU).
 int resolve_threads(int threads) {
     if (threads > 0) return threads;
-    const unsigned hc = std::thread::hardware_concurrency();
-    return hc ? (int)hc : 1;
+    int n = 0;
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+        const int cap = std::atoi(e);
+        if (cap > 0 && (n <= 0 || cap < n)) n = cap;
+    }
+    return n > 0 ? n : 1;
 }
 
 uint32_t spread3(uint32_t v) {
@@ -646,6 +657,8 @@ bool cpu_bucket(or_cpu *h, const sp_view *view, uint32_t w, uint32_t hh, int thr
 }  // namespace
 
 extern "C" {
+
+int or_resolve_threads(int threads) { return resolve_threads(threads); }
 
 // Prep::new (src/render/cpu.rs:349-393).
 or_cpu *or_cpu_create(const sp_scene *scene) {

@@ -193,6 +193,11 @@ HIP_SYMBOLS = {
         C.c_int,
         [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, u64p, u32p, u32p, C.c_uint64, u64p],
     ),
+    "sp_debug_split_pairs": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, u32p, C.c_uint64, u64p,
+         u64p],
+    ),
     "sp_debug_reach": (C.c_int, [f32p, u16p, C.c_uint32, C.c_uint32, u8p]),
     "sp_last_frame_info": (C.c_int, [C.c_void_p, C.POINTER(SpFrameInfo)]),
     "sp_render_batch": (

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -267,6 +268,8 @@ int ensure_table(sp_handle *h) {
                                     sp::radix_table_words(1, std::max<uint64_t>(h->pair_cap, 1)),
                                     sp::radix_table_bound(vc)});
     if (need <= h->table_words) return SP_OK;
+    if (h->opt.max_device_bytes && h->bytes - h->table_words * 4 + need * 4 > h->opt.max_device_bytes)
+        return fail(SP_E_NOMEM, "device memory budget exceeded growing the radix table");
     dfree(h, h->d_table, h->table_words * 4);
     h->table_words = 0;
     const int rc = dalloc(h, (void **)&h->d_table, need * 4);
@@ -278,6 +281,12 @@ int ensure_table(sp_handle *h) {
 int ensure_pairs(sp_handle *h, uint64_t cap) {
     if (cap <= h->pair_cap) return SP_OK;
     if (cap > 0xFFFFFFFFull) return fail(SP_E_CAPACITY, "more than 2^32 gaussian/tile pairs");
+    // The budget is checked before anything is freed: a refused growth leaves
+    // the handle's buffers (and its ability to render smaller frames) intact.
+    const uint64_t per_pair = (h->diff ? 5ull : 4ull) * 4;
+    if (h->opt.max_device_bytes && h->bytes - h->pair_cap * per_pair + cap * per_pair > h->opt.max_device_bytes)
+        return fail(SP_E_NOMEM, "device memory budget exceeded growing the pair buffers to " + std::to_string(cap) +
+                                    " pairs");
     for (int b = 0; b < 2; ++b) {
         dfree(h, h->d_pkey[b], h->pair_cap * 4);
         dfree(h, h->d_pval[b], h->pair_cap * 4);
@@ -499,8 +508,10 @@ int setup(sp_handle *h) {
 // BL_RGB with a split runs the emission, tile sort, ranges and blend twice:
 // the first phase for each view's nearest splats over all tiles, the second
 // for the rest over the tiles the first blend left unfinished.
+// `after_first` (inspection only, never captured) runs after the first blend
+// with the first phase's lists in d_pval[first list buffer].
 int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_sbs, int tile_bits, int mode,
-            bool events, int *pair_buf) {
+            bool events, int *pair_buf, const std::function<int(int)> *after_first = nullptr) {
     const sp::DevScene sc = dev_scene(h);
     hipStream_t s = h->stream;
     uint32_t *vis = h->d_vstat;
@@ -553,6 +564,10 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
                                  h->blend_batch));
     }
     HIP_TRY(mark(3));
+    if (after_first) {
+        const int rc = (*after_first)(pi);
+        if (rc) return rc;
+    }
     if (split) {
         HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip, vopen));
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
@@ -583,7 +598,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
 // BL_DIFF as device float images (the differentiable forward), BL_NONE stops
 // after the tile ranges. use_graph: capture the batch once and replay it.
 int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_t *wh, const float bg[3],
-              void *const *outs, int mode, bool use_graph, sp_stats *stats) {
+              void *const *outs, int mode, bool use_graph, sp_stats *stats,
+              const std::function<int(int)> *after_first = nullptr) {
     const bool blend = mode != BL_NONE;
     const double t0 = now_ms();
     if (nviews == 0) return SP_OK;
@@ -591,6 +607,10 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     if ((uint64_t)nviews * std::max<uint32_t>(h->n, 1) > 0xFFFFFFFFull)
         return fail(SP_E_INVALID, "views x gaussians exceeds 2^32 records");
     int rc = ensure_views(h, nviews);
+    // A failed growth (a device allocation that did not succeed) leaves no pair
+    // buffer; the kernels always get one.
+    if (!rc && h->pair_cap == 0) rc = ensure_pairs(h, 1ull << 20);
+    if (!rc) rc = ensure_table(h);  // after a failed growth the pairs may have grown without their table
     if (rc) return rc;
     uint64_t total = 0, total_sb = 0;
     for (uint32_t v = 0; v < nviews; ++v) {
@@ -651,7 +671,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             HIP_TRY(hipGraphLaunch(ge.exec, h->stream));
             pb = ge.pair_buf;
         } else {
-            rc = enqueue(h, nviews, total_tiles, (uint32_t)total_sb, tile_bits, mode, events, &pb);
+            rc = enqueue(h, nviews, total_tiles, (uint32_t)total_sb, tile_bits, mode, events, &pb, after_first);
             if (rc) return rc;
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1048,6 +1068,57 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     HIP_TRY(hipMemcpy(dv.data(), h->d_dval[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
     for (uint32_t q = 0; q < nv; ++q) depth[dv[q]] = dk[q];
     for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | depth[vals[p]];
+    return SP_OK;
+}
+
+int sp_debug_split_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, uint32_t *first_vals,
+                         uint32_t *first_start, uint32_t *done, uint32_t *second_vals, uint32_t *second_start,
+                         uint64_t cap, uint64_t *n_first, uint64_t *n_second) {
+    if (!h || !view || !first_start || !done || !second_start || !n_first || !n_second)
+        return fail(SP_E_INVALID, "sp_debug_split_pairs: NULL argument");
+    *n_first = *n_second = 0;
+    int rc = check_frame(width, height);
+    if (rc) return rc;
+    if (width == 0 || height == 0) return SP_OK;
+    if (h->diff) return fail(SP_E_INVALID, "sp_debug_split_pairs: a differentiable handle has no split");
+    rc = setup(h);
+    if (rc) return rc;
+    rc = ensure_rgb(h, 3ull * width * height);
+    if (rc) return rc;
+    const uint32_t tiles = ((width + 15) / 16) * ((height + 15) / 16);
+    const bool split = h->split_shift > 0;
+    std::vector<uint32_t> v1;
+    // Copy the first phase's lists and tile flags out before the second phase
+    // reuses the buffers.
+    const std::function<int(int)> grab = [&](int pi) -> int {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipMemcpy(first_start, h->d_tile_start, (tiles + 1ull) * 4, hipMemcpyDeviceToHost));
+        const uint64_t n1 = first_start[tiles];
+        v1.resize(n1);
+        if (n1) HIP_TRY(hipMemcpy(v1.data(), h->d_pval[pi], n1 * 4, hipMemcpyDeviceToHost));
+        if (split)
+            HIP_TRY(hipMemcpy(done, h->d_done, tiles * 4ull, hipMemcpyDeviceToHost));
+        else
+            std::memset(done, 0, tiles * 4ull);
+        return SP_OK;
+    };
+    const uint32_t wh[2] = {width, height};
+    const float bg[3] = {0.f, 0.f, 0.f};
+    void *outs[1] = {h->d_rgb};
+    rc = run_views(h, 1, view, wh, bg, outs, BL_RGB, false, nullptr, &grab);
+    if (rc) return rc;
+    *n_first = v1.size();
+    if (split) {
+        HIP_TRY(hipMemcpy(second_start, h->d_tile_start, (tiles + 1ull) * 4, hipMemcpyDeviceToHost));
+        *n_second = second_start[tiles];
+    } else {
+        std::memset(second_start, 0, (tiles + 1ull) * 4);
+    }
+    if (*n_first > cap || *n_second > cap) return fail(SP_E_CAPACITY, "sp_debug_split_pairs: capacity too small");
+    if ((*n_first && !first_vals) || (*n_second && !second_vals))
+        return fail(SP_E_INVALID, "sp_debug_split_pairs: NULL value buffer");
+    if (*n_first) std::memcpy(first_vals, v1.data(), *n_first * 4);
+    if (*n_second) HIP_TRY(hipMemcpy(second_vals, h->d_pval[h->pair_buf], *n_second * 4, hipMemcpyDeviceToHost));
     return SP_OK;
 }
 

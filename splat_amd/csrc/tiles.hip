@@ -4,18 +4,17 @@
 // splat's clamped pixel box into its 16x16 tile rectangle (the tiles
 // [x0/16, ceil(x1/16)) x [y0/16, ceil(y1/16)) of src/render/cpu.rs:266-276; an
 // empty box gives no pairs, :262-264). Offsets come from a reduce-then-scan:
-// a count pass writes each block's pair total, one scan workgroup per view
-// turns them into block offsets (row totals = pairs per view), a one-row scan
-// turns the view totals into view bases, and the expand pass re-derives the
-// rectangles and writes the pairs, each thread finding its splat by binary
-// search over in-block offsets. The dense expansion deals fixed-size chunks
-// of pairs to workgroups (the nearest splats cover thousands of tiles, so
-// per-splat-block work would be badly unbalanced); the sparse second phase
-// keeps one workgroup per splat block. Pairs come out view by view in
-// (depth, scene index) order, carrying a global tile id (view tile base +
-// tile) and a record index (view * n + scene index), ready for the stable
-// tile-id sort. The first phase is binned instead of sorted (superblock
-// pairs, then per-superblock tile lists; see "Binned first phase" below).
+// a count pass writes each block's pair totals, one scan workgroup per view
+// turns them into block offsets (row totals = pairs per view), and a one-row
+// scan turns the view totals into view bases.
+// The first phase is binned (see "Binned first phase" below): a
+// pair-balanced expansion writes one pair per (splat, 4x4-tile superblock)
+// its rectangle meets, the superblock pairs are sorted stably by superblock,
+// and per-superblock workgroups write each tile's list in (depth, scene
+// index) order. The sparse second phase expands tile pairs with one
+// workgroup per splat block, each carrying a global tile id (view tile base
+// + tile) and a record index (view * n + scene index), ready for the stable
+// tile-id sort.
 // The differentiable path also numbers the tile pairs in emission order: the
 // record of each emission slot (powner) and each record's first slot
 // (pbase), so that per-pair gradients can be written back in emission order,
@@ -45,10 +44,10 @@
 namespace sp {
 namespace {
 
-constexpr uint32_t EXPAND_CHUNK = 4096;  // pairs per workgroup step of the expansion
-// Superblock pairs are ~20x fewer than tile pairs: their chunk follows the
+// Superblock pairs per workgroup step of the expansion: the chunk follows the
 // live count (512 to 4096 pairs), so few pairs still spread over many CUs.
-constexpr uint32_t EXPAND_CHUNK_BIN_MIN = 512;
+constexpr uint32_t EXPAND_CHUNK_MAX = 4096;
+constexpr uint32_t EXPAND_CHUNK_MIN = 512;
 constexpr uint32_t EXPAND_GRID = 2048;   // workgroups of the expansion (grid-stride over chunks)
 
 struct Rect {
@@ -210,21 +209,19 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *_
     box[vbase + j] = sbox[vbase + dval[vbase + j]];
 }
 
-// Pair-balanced expansion: the phase's pairs (views concatenated) are cut
-// into EXPAND_CHUNK-pair chunks dealt to the workgroups; a workgroup finds the
-// splat block holding its chunk's first pair (binary search over the view
-// bases and the view's block offsets), re-derives that block's 256
-// rectangles and their in-block offsets, expands the part of the chunk they
-// cover, and moves on to the next block until the chunk is done. The nearest
-// splats of a view cover thousands of tiles each, so splat-major work would
-// leave one workgroup with most of the pairs.
-//
-// BIN: the binned first phase expands each splat's superblock rectangle
-// instead: key = global superblock id | sub-rectangle << SB_SUB_SHIFT (the
+// Pair-balanced expansion of the first phase's superblock pairs: the
+// phase's pairs (views concatenated) are cut into chunks of 512-4096 pairs
+// dealt to the workgroups; a workgroup finds the splat block holding its
+// chunk's first pair (binary search over the view bases and the view's block
+// offsets), re-derives that block's 256 superblock rectangles and their
+// in-block offsets, expands the part of the chunk they cover, and moves on to
+// the next block until the chunk is done. The nearest splats of a view cover
+// hundreds of superblocks each, so splat-major work would leave one workgroup
+// with most of the pairs.
+// A pair is key = global superblock id | sub-rectangle << SB_SUB_SHIFT (the
 // part of the splat's tile rectangle inside the superblock, sb_sub), value =
 // record index; the pair count comes from counters[CNT_SBPAIRS].
-template <bool BIN>
-__global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32_t *__restrict__ dval, uint32_t n_splats,
+__global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_sb(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                               uint32_t bpv, uint32_t nviews,
                                                               const uint32_t *__restrict__ vis_count,
                                                               const uint2 *__restrict__ box,
@@ -237,19 +234,17 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
     __shared__ uint32_t s_org[EMIT_THREADS];  // first global tile id of the rectangle
     __shared__ uint32_t s_rw[EMIT_THREADS];   // rectangle width in tiles
     __shared__ uint32_t s_id[EMIT_THREADS];   // record index (view * n + scene index)
-    __shared__ uint32_t s_t0[BIN ? EMIT_THREADS : 1], s_t1[BIN ? EMIT_THREADS : 1];  // tile rectangle (BIN)
+    __shared__ uint32_t s_t0[EMIT_THREADS], s_t1[EMIT_THREADS];  // tile rectangle
     __shared__ uint32_t s_v, s_blk;
-    const uint64_t np = counters[BIN ? CNT_SBPAIRS : CNT_PAIRS];
+    const uint64_t np = counters[CNT_SBPAIRS];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // The tile pairs (which a binned list holds too) set the capacity need.
         counters[CNT_OVERFLOW] = counters[CNT_PAIRS] > pair_cap ? 1u : 0u;
         counters[CNT_STAGED] = 0u;  // the blends of this frame accumulate into it
     }
     const uint64_t nw = min(np, pair_cap);
-    const uint32_t CHUNK =
-        BIN ? (uint32_t)min((uint64_t)EXPAND_CHUNK,
-                            max((uint64_t)EXPAND_CHUNK_BIN_MIN, ((nw / gridDim.x + 255) / 256) * 256))
-            : EXPAND_CHUNK;
+    const uint32_t CHUNK = (uint32_t)min((uint64_t)EXPAND_CHUNK_MAX,
+                                         max((uint64_t)EXPAND_CHUNK_MIN, ((nw / gridDim.x + 255) / 256) * 256));
     for (uint64_t c0 = (uint64_t)blockIdx.x * CHUNK; c0 < nw; c0 += (uint64_t)gridDim.x * CHUNK) {
         const uint64_t c1 = min(c0 + CHUNK, nw);
         uint64_t q = c0;
@@ -286,24 +281,17 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
             const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
             const uint32_t j = blk * EMIT_THREADS + threadIdx.x;
             const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
-            const uint32_t gw = BIN ? (tiles_x + SB - 1) / SB : tiles_x;  // grid width of the emitted cells
-            uint32_t cnt = phase_count<EMIT_FIRST>(r, j, k1, nullptr, tile_base, tiles_x), org = r.org, rw = r.rw;
-            if (BIN) {
-                const SbRect q = sb_rect(r, gw, views[v].sb_base);
-                cnt = j < k1 ? q.cnt : 0u;
-                org = q.org;
-                rw = q.rw;
-            }
+            const uint32_t gw = (tiles_x + SB - 1) / SB;  // superblock grid width
+            const SbRect sq = sb_rect(r, gw, views[v].sb_base);
+            const uint32_t cnt = j < k1 ? sq.cnt : 0u, org = sq.org, rw = sq.rw;
             uint32_t total;
             const uint32_t lscan = block_exclusive_scan<EMIT_THREADS>(cnt, &total);
             s_pre[threadIdx.x] = lscan;
             s_org[threadIdx.x] = org;
             s_rw[threadIdx.x] = rw;
             s_id[threadIdx.x] = r.id;
-            if (BIN) {
-                s_t0[threadIdx.x] = r.tx0 | (r.ty0 << 16);
-                s_t1[threadIdx.x] = r.tx1 | (r.ty1 << 16);
-            }
+            s_t0[threadIdx.x] = r.tx0 | (r.ty0 << 16);
+            s_t1[threadIdx.x] = r.tx1 | (r.ty1 << 16);
             __syncthreads();
             const uint64_t bbase = (uint64_t)vbase_pairs[v] + boff[(uint64_t)v * bpv + blk];
             const uint64_t hi = min(c1, bbase + total);
@@ -322,14 +310,9 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_dense(const uint32
                 const uint32_t rwl = s_rw[lo];
                 const uint32_t ry = loc / rwl, rx = loc - ry * rwl;
                 const uint32_t tile = s_org[lo] + ry * gw + rx;
-                if (BIN) {
-                    const uint32_t t0 = s_t0[lo], t1 = s_t1[lo];
-                    const uint32_t bx = ((t0 & 0xFFFFu) / SB + rx) * SB, by = ((t0 >> 16) / SB + ry) * SB;
-                    pkey[qq] = tile | (sb_sub(t0 & 0xFFFFu, t0 >> 16, t1 & 0xFFFFu, t1 >> 16, bx, by) << SB_SUB_SHIFT);
-                    pval[qq] = s_id[lo];
-                    continue;
-                }
-                pkey[qq] = tile;
+                const uint32_t t0 = s_t0[lo], t1 = s_t1[lo];
+                const uint32_t bx = ((t0 & 0xFFFFu) / SB + rx) * SB, by = ((t0 >> 16) / SB + ry) * SB;
+                pkey[qq] = tile | (sb_sub(t0 & 0xFFFFu, t0 >> 16, t1 & 0xFFFFu, t1 >> 16, bx, by) << SB_SUB_SHIFT);
                 pval[qq] = s_id[lo];
             }
             q = hi;
@@ -790,9 +773,9 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     if (e == hipSuccess) e = launch_scan_rows_multi(s, bin ? 2 : 1, view_data, view_tot, nviews, 1);
     if (e != hipSuccess) return e;
     if (bin) {
-        const uint64_t chunks = (pair_cap + EXPAND_CHUNK_BIN_MIN - 1) / EXPAND_CHUNK_BIN_MIN;
+        const uint64_t chunks = (pair_cap + EXPAND_CHUNK_MIN - 1) / EXPAND_CHUNK_MIN;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
-        hipLaunchKernelGGL(k_emit_expand_dense<true>, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+        hipLaunchKernelGGL(k_emit_expand_sb, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, pair_cap, counters);
         if (powner) {  // the differentiable path also numbers the tile pairs in emission order
             const hipError_t eb = hipGetLastError();

@@ -224,6 +224,25 @@ class HipRenderer:
         return keys[:k], vals[:k], ts
 
 
+    def debug_split_pairs(self, view: N.SpView, width: int, height: int, cap: int | None = None):
+        """Both phases' tile lists of a real split render: (first_vals, first_start,
+        done, second_vals, second_start); values are scene indices in blend order."""
+        tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        fs = np.zeros(tiles + 1, np.uint32)
+        ss = np.zeros(tiles + 1, np.uint32)
+        done = np.zeros(tiles, np.uint32)
+        n1, n2 = C.c_uint64(0), C.c_uint64(0)
+        cap = max(1, self.n * 8 if cap is None else cap)
+        fv = np.zeros(cap, np.uint32)
+        sv = np.zeros(cap, np.uint32)
+        u32 = lambda a: a.ctypes.data_as(N.u32p)  # noqa: E731
+        rc = self._lib.sp_debug_split_pairs(self._h, C.byref(view), width, height, u32(fv), u32(fs), u32(done),
+                                            u32(sv), u32(ss), cap, C.byref(n1), C.byref(n2))
+        if rc == N.SP_E_CAPACITY:
+            return self.debug_split_pairs(view, width, height, int(max(n1.value, n2.value)))
+        N.check(rc)
+        return fv[:n1.value], fs, done.astype(bool), sv[:n2.value], ss
+
 def debug_reach(proj, bbox, width: int, height: int) -> np.ndarray:
     """The blend's quadrant reach bits of one projected splat (a row of
     debug_project's proj and bbox) for every 16x16 tile of a width x height

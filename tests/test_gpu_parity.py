@@ -330,6 +330,10 @@ def test_more_than_2_32_pairs(oracle):
     with pytest.raises(N.SplatError) as e:
         r.render(v, 4096, 4096, (0, 0, 0))
     assert e.value.code == N.SP_E_CAPACITY and "2^32" in str(e.value)
+    case("2^32 pairs: the handle renders a later frame")
+    small, _ = r.render(v, 64, 64, (0, 0, 0))
+    ref, _ = oracle.CpuTiled(s).render(v, 64, 64, (0, 0, 0))
+    assert np.array_equal(small.pixels, ref)
     r.close()
 
 
