@@ -130,6 +130,27 @@ int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height
 int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
                      const float background[3], uint8_t *rgb_out_device, sp_stats *stats);
 
+/* Bytes of the static per-frame scratch of sp_render_device_static, reserved
+ * on every device the library's code object loads on. */
+#define SP_STATIC_SCRATCH_BYTES (128ull << 20)
+
+/* sp_render_device without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate (code between HIP graph
+ * captures, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-frame buffers (records, sort keys, tile pairs, tables, tile state) are
+ * pieces of one static region of SP_STATIC_SCRATCH_BYTES per device instead
+ * of the handle's growable buffers; the scene stays the handle's (uploaded by
+ * sp_create). The per-splat arrays take 80 bytes per splat and, with the
+ * front-to-back split on, a 1920x1080 frame's tile state 34 MB; the tile
+ * pairs of a phase get what is left, 16 bytes each. A frame that does not
+ * fit returns SP_E_CAPACITY (the output's contents are then unspecified) and
+ * the handle stays usable. Images and stats equal sp_render_device's. Every
+ * call on a GPU shares that GPU's region: calls that overlap on one GPU (from
+ * several threads or handles) race; each call returns after its frame is
+ * complete, so calls in sequence are safe. */
+int sp_render_device_static(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+                            const float background[3], uint8_t *rgb_out_device, sp_stats *stats);
+
 /* Last error message of the calling thread ("" if none). */
 const char *sp_last_error(void);
 

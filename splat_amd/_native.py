@@ -182,6 +182,10 @@ HIP_SYMBOLS = {
         C.c_int,
         [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, f32p, C.c_void_p, C.POINTER(SpStats)],
     ),
+    "sp_render_device_static": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(SpView), C.c_uint32, C.c_uint32, f32p, C.c_void_p, C.POINTER(SpStats)],
+    ),
     "sp_last_error": (C.c_char_p, []),
     "sp_downsample_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "sp_render_cells": (

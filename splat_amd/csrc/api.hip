@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,12 @@
 
 using sp::ViewCtx;
 using sp::ViewParams;
+
+// Per-frame scratch of sp_render_device_static: one region per GPU, reserved
+// when the library's code object loads on that device and held until the
+// process ends. Host code must not use this name as an address (it is a host
+// stand-in); it uses the per-device pointer from hipGetSymbolAddress.
+static __device__ __align__(256) uint8_t scratch[SP_STATIC_SCRATCH_BYTES];
 
 namespace {
 
@@ -199,6 +206,9 @@ struct sp_handle {
     uint64_t part_cap = 0;
     uint32_t fwd_views = 0, fwd_tiles = 0;  // shape of the last differentiable forward
     uint64_t fwd_pairs = 0;
+    // Buffers carved from a fixed region (sp_render_device_static): nothing may
+    // grow or be freed; a frame that does not fit is SP_E_CAPACITY.
+    bool fixed = false;
     // Captured pipelines, by shape.
     GraphEntry graphs[GRAPH_CACHE];
     uint64_t graph_clock = 0;
@@ -232,6 +242,12 @@ void dfree(sp_handle *h, T *&p, uint64_t bytes) {
     p = nullptr;
 }
 
+// A fixed handle's buffers cannot grow.
+int fixed_full(const char *what) {
+    return fail(SP_E_CAPACITY, std::string("the frame's ") + what + " exceed the static scratch of sp_render_device_static (" +
+                                   std::to_string(SP_STATIC_SCRATCH_BYTES >> 20) + " MiB)");
+}
+
 void drop_graph(GraphEntry &g) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -262,8 +278,15 @@ GraphEntry &graph_slot(sp_handle *h, const GraphKey &key, bool *hit) {
 
 // Radix count table large enough for the depth sort of view_cap views and the
 // tile sort of pair_cap pairs.
+// The table of the depth sort (one segment per view) and of a pair sort, the
+// two sizes a fixed handle's single-view sorts use.
+uint64_t table_need_single(uint64_t n, uint64_t pairs) {
+    return std::max(sp::radix_table_words(1, std::max<uint64_t>(n, 1)), sp::radix_table_words(1, std::max<uint64_t>(pairs, 1)));
+}
+
 int ensure_table(sp_handle *h) {
     const uint32_t vc = std::max<uint32_t>(h->view_cap, 1);
+    if (h->fixed) return table_need_single(h->n, h->pair_cap) <= h->table_words ? SP_OK : fixed_full("radix tables");
     const uint64_t need = std::max({sp::radix_table_words(vc, std::max<uint32_t>(h->n, 1)),
                                     sp::radix_table_words(1, std::max<uint64_t>(h->pair_cap, 1)),
                                     sp::radix_table_bound(vc)});
@@ -280,6 +303,7 @@ int ensure_table(sp_handle *h) {
 
 int ensure_pairs(sp_handle *h, uint64_t cap) {
     if (cap <= h->pair_cap) return SP_OK;
+    if (h->fixed) return fixed_full("tile pairs");
     if (cap > 0xFFFFFFFFull) return fail(SP_E_CAPACITY, "more than 2^32 gaussian/tile pairs");
     // The budget is checked before anything is freed: a refused growth leaves
     // the handle's buffers (and its ability to render smaller frames) intact.
@@ -313,6 +337,7 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
 // Per-view arrays for `views` views.
 int ensure_views(sp_handle *h, uint32_t views) {
     if (views <= h->view_cap) return SP_OK;
+    if (h->fixed) return fixed_full("views");
     const uint64_t nn = std::max<uint64_t>(h->n, 1);
     const uint64_t old = h->view_cap;
     const uint64_t blocks = std::max(sp::project_blocks(h->n), sp::emit_blocks(h->n));
@@ -368,6 +393,7 @@ int ensure_views(sp_handle *h, uint32_t views) {
 
 int ensure_tiles(sp_handle *h, uint64_t tiles) {
     if (tiles + 1 <= h->tile_cap) return SP_OK;
+    if (h->fixed) return fixed_full("tiles");
     dfree(h, h->d_tile_start, h->tile_cap * 4);
     h->tile_cap = 0;
     const int rc = dalloc(h, (void **)&h->d_tile_start, (tiles + 1) * 4);
@@ -378,6 +404,8 @@ int ensure_tiles(sp_handle *h, uint64_t tiles) {
 
 // Superblock bounds and tile counts of the binned first phase.
 int ensure_bin(sp_handle *h, uint64_t tiles, uint64_t sbs) {
+    if (h->fixed && (sbs + 1 > h->sb_cap || sp::bin_scan_rows((uint32_t)tiles) * sp::BIN_SCAN_ROW > h->tcount_cap))
+        return fixed_full("superblocks");
     if (sbs + 1 > h->sb_cap) {
         const uint64_t seg_words = (uint64_t)sp::BIN_SEGS * sp::SB * sp::SB;
         dfree(h, h->d_sb_start, h->sb_cap * 4);
@@ -408,6 +436,7 @@ int ensure_bin(sp_handle *h, uint64_t tiles, uint64_t sbs) {
 int ensure_pix(sp_handle *h, uint64_t tiles) {
     const uint64_t need = tiles * sp::BLEND_THREADS;
     if (need <= h->pix_cap) return SP_OK;
+    if (h->fixed) return fixed_full("pixels");
     dfree(h, h->d_pixT, h->pix_cap * 4);
     dfree(h, h->d_pixlast, h->pix_cap * 4);
     h->pix_cap = 0;
@@ -421,6 +450,7 @@ int ensure_pix(sp_handle *h, uint64_t tiles) {
 // Tile flags, row prefixes and pixel state of the front-to-back split.
 int ensure_split(sp_handle *h, uint64_t tiles) {
     if (tiles <= h->split_cap) return SP_OK;
+    if (h->fixed) return fixed_full("tile states");
     dfree(h, h->d_done, h->split_cap * 4);
     dfree(h, h->d_ip, h->split_cap * 4);
     dfree(h, h->d_pstate, h->split_cap * sp::BLEND_THREADS * 16);
@@ -435,6 +465,7 @@ int ensure_split(sp_handle *h, uint64_t tiles) {
 
 // Gradient scratch of the backward: per-pair, per-record and carry arrays.
 int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
+    if (h->fixed) return fixed_full("gradients");
     int rc = SP_OK;
     if (pairs > h->gpair_cap) {
         dfree(h, h->d_gpair, h->gpair_cap * sp::NG2D * 4);
@@ -467,6 +498,7 @@ int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
 
 int ensure_rgb(sp_handle *h, uint64_t bytes) {
     if (bytes <= h->rgb_cap) return SP_OK;
+    if (h->fixed) return fixed_full("frame bytes");
     dfree(h, h->d_rgb, h->rgb_cap);
     h->rgb_cap = 0;
     const int rc = dalloc(h, (void **)&h->d_rgb, bytes);
@@ -819,6 +851,90 @@ int device_ok(const sp_options &opt) {
     return SP_OK;
 }
 
+// Device address of `scratch` on the current device (h->device), looked up
+// once per device.
+int static_scratch(int device, uint8_t **out) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static uint8_t *ptr[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return fail(SP_E_DEVICE, "device ordinal out of range for the static scratch");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SP_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr[device] = (uint8_t *)p;
+    }
+    *out = ptr[device];
+    return SP_OK;
+}
+
+// Bump allocation of 256-aligned pieces of a region; with base == nullptr it
+// only measures.
+struct Carve {
+    uint8_t *base;
+    uint64_t off = 0;
+    template <class T>
+    T *take(uint64_t bytes) {
+        off = (off + 255) & ~255ull;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += std::max<uint64_t>(bytes, 16);
+        return p;
+    }
+};
+
+// Points a copy of the handle's per-frame buffers at pieces of `base` for one
+// view of width x height with pair capacity `pairs`; returns the bytes used.
+uint64_t carve_frame(sp_handle *t, uint8_t *base, uint32_t width, uint32_t height, uint64_t pairs) {
+    Carve c{base};
+    const uint64_t n = std::max<uint32_t>(t->n, 1);
+    const uint64_t blocks = std::max(sp::project_blocks(t->n), sp::emit_blocks(t->n));
+    const uint64_t tiles = (uint64_t)((width + 15) / 16) * ((height + 15) / 16);
+    const uint64_t sbs = (uint64_t)((width + 16 * sp::SB - 1) / (16 * sp::SB)) * ((height + 16 * sp::SB - 1) / (16 * sp::SB));
+    const uint64_t rows = sp::bin_scan_rows((uint32_t)tiles);
+    t->view_cap = 1;
+    t->d_rec = c.take<uint4>(n * 48);
+    t->d_box = c.take<uint2>(n * 8);
+    t->d_sbox = c.take<uint2>(n * 8);
+    for (int b = 0; b < 2; ++b) {
+        t->d_dkey[b] = c.take<uint32_t>(n * 4);
+        t->d_dval[b] = c.take<uint32_t>(n * 4);
+    }
+    t->d_statblk = c.take<uint32_t>(sp::CNT_SLOTS * 4 + 20);
+    t->d_counters = t->d_statblk;
+    t->d_vstat = t->d_statblk ? t->d_statblk + sp::CNT_SLOTS : nullptr;
+    t->d_bcnt = c.take<uint32_t>(blocks * 4);
+    t->d_bcnt_all = c.take<uint32_t>(blocks * 4);
+    t->d_bsb = c.take<uint32_t>(blocks * 4);
+    t->d_views = c.take<ViewParams>(sizeof(ViewParams));
+    t->d_rowtot = c.take<uint32_t>(sp::RADIX_BINS_MAX * 4);
+    t->table_words = table_need_single(t->n, pairs);
+    t->d_table = c.take<uint32_t>(t->table_words * 4);
+    t->pair_cap = pairs;
+    for (int b = 0; b < 2; ++b) {
+        t->d_pkey[b] = c.take<uint32_t>(pairs * 4);
+        t->d_pval[b] = c.take<uint32_t>(pairs * 4);
+    }
+    t->d_powner = nullptr;
+    t->tile_cap = tiles + 1;
+    t->d_tile_start = c.take<uint32_t>((tiles + 1) * 4);
+    t->sb_cap = sbs + 1;
+    t->d_sb_start = c.take<uint32_t>((sbs + 1) * 4);
+    t->d_segc = c.take<uint32_t>(sbs * sp::BIN_SEGS * sp::SB * sp::SB * 4);
+    t->d_sbtile = c.take<uint32_t>(sbs * sp::SB * sp::SB * 4);
+    t->tcount_cap = rows * sp::BIN_SCAN_ROW;
+    t->d_tcount = c.take<uint32_t>(rows * sp::BIN_SCAN_ROW * 4);
+    t->d_trow = c.take<uint32_t>(rows * 4);
+    t->split_cap = t->split_shift > 0 ? tiles : 0;
+    t->d_done = t->split_shift > 0 ? c.take<uint32_t>(tiles * 4) : nullptr;
+    t->d_ip = t->split_shift > 0 ? c.take<uint32_t>(tiles * 4) : nullptr;
+    t->d_pstate = t->split_shift > 0 ? c.take<float>(tiles * sp::BLEND_THREADS * 16) : nullptr;
+    t->rgb_cap = 0;
+    t->d_rgb = nullptr;
+    return c.off;
+}
+
 }  // namespace
 
 extern "C" {
@@ -938,6 +1054,46 @@ int sp_render_device(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     rc = setup(h);
     if (rc) return rc;
     return render_one(h, view, width, height, background, rgb_out_device, stats);
+}
+
+int sp_render_device_static(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height,
+                            const float background[3], uint8_t *rgb_out_device, sp_stats *stats) {
+    if (!h || !view || !background) return fail(SP_E_INVALID, "sp_render_device_static: NULL argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    int rc = check_frame(width, height);
+    if (rc) return rc;
+    if (width == 0 || height == 0) return SP_OK;
+    if (!rgb_out_device) return fail(SP_E_INVALID, "sp_render_device_static: NULL output");
+    if (h->diff) return fail(SP_E_INVALID, "sp_render_device_static: not for a differentiable handle");
+    rc = setup(h);
+    if (rc) return rc;
+    uint8_t *base = nullptr;
+    rc = static_scratch(h->device, &base);
+    if (rc) return rc;
+    // A copy of the handle whose per-frame buffers are pieces of the scratch:
+    // the scene, stream, events, tuning and pinned host staging are the
+    // handle's own (all made by sp_create), and nothing is captured.
+    sp_handle t = *h;
+    for (GraphEntry &g : t.graphs) g = GraphEntry{};
+    t.fixed = true;
+    // The pair capacity takes what the fixed buffers leave.
+    const uint64_t fixed_bytes = carve_frame(&t, nullptr, width, height, 0);
+    if (fixed_bytes + 256 * 4 > SP_STATIC_SCRATCH_BYTES)
+        return fixed_full("per-splat and per-tile arrays");
+    uint64_t pairs = (SP_STATIC_SCRATCH_BYTES - fixed_bytes) / 16;
+    while (pairs > 0 && carve_frame(&t, nullptr, width, height, pairs) > SP_STATIC_SCRATCH_BYTES) pairs -= pairs / 32 + 1;
+    pairs = std::min<uint64_t>(pairs, 0xFFFFFFFFull);
+    if (pairs == 0) return fixed_full("tile pairs");
+    carve_frame(&t, base, width, height, pairs);
+    // The frame counters start at 0 (a fill kernel: the call may sit between
+    // captured work of the caller).
+    HIP_TRY(sp::launch_fill(t.stream, t.d_statblk, sp::CNT_SLOTS + 5, 0u));
+    const uint32_t wh[2] = {width, height};
+    void *outs[1] = {rgb_out_device};
+    rc = run_views(&t, 1, view, wh, background, outs, BL_RGB, false, stats);
+    h->last = t.last;
+    h->last.device_bytes = h->bytes;
+    return rc;
 }
 
 int sp_render(sp_handle *h, const sp_view *view, uint32_t width, uint32_t height, const float background[3],
