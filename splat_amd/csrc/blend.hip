@@ -49,7 +49,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                                                          uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
                                                          float4 *__restrict__ pstate) {
     __shared__ uint4 s_rec[BATCH][3];
-    __shared__ uint32_t s_qm[BATCH];  // quadrant_mask of each staged record
+    __shared__ uint32_t s_qm[BATCH];  // reach bits (0-3) and full-cover bits (4-7) of each staged record
     // Consecutive tiles (rows of an image) on the same XCD share its L2.
     const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
     if (gt >= total_tiles) return;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
             const uint4 *r = rec + 3 * (uint64_t)id;
             uint4 r0 = r[0];
             const uint4 r1 = r[1], r2 = r[2];
-            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy);
+            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy) | (cover_mask(r2, ox, oy) << 4);
             // The staged x is the tile's ((ox + 0.5) - x), the reference's
             // per-record term of every pixel offset, computed once here.
             r0.x = __float_as_uint((fx + 0.5f) - __uint_as_float(r0.x));
@@ -110,22 +110,34 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         // time: one LDS read per lane and a ballot, then the set bits in order.
         bool open = live_mask() != 0ull;
         for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
-            unsigned long long reach = __ballot(g0 + lane_i < nb && ((s_qm[g0 + lane_i] >> w) & 1u));
+            const uint32_t qm = g0 + lane_i < nb ? s_qm[g0 + lane_i] : 0u;
+            unsigned long long reach = lane_mask((qm >> w) & 1u);
+            // Records whose box holds the whole quadrant: no box test.
+            const unsigned long long cover = lane_mask((qm >> (4 + w)) & 1u);
             while (reach) {
                 const uint64_t lm = live_mask();
                 if (lm == 0ull) {  // every pixel of the quadrant saturated
                     open = false;
                     break;
                 }
-                const uint32_t k = g0 + (uint32_t)__builtin_ctzll(reach);
+                const uint32_t kb = (uint32_t)__builtin_ctzll(reach);
+                const uint32_t k = g0 + kb;
                 reach &= reach - 1ull;
-                const uint4 r2 = s_rec[k][2];
-                const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-                const bool act = live() && px >= bx0 && px < bx1 && py >= by0 && py < by1;
-                const uint64_t act_mask =
-                    lm & lane_mask(px >= bx0) & lane_mask(px < bx1) & lane_mask(py >= by0) & lane_mask(py < by1);
                 const uint4 r0 = s_rec[k][0];
                 const uint4 r1 = s_rec[k][1];
+                const uint4 r2 = s_rec[k][2];
+                // All 48 bytes are read at once from one address register (the
+                // compiler would otherwise sink each field's read to its first
+                // use and rebuild the address for each, one VALU move apiece).
+                asm volatile("" ::"v"(r0.x), "v"(r0.y), "v"(r0.z), "v"(r0.w), "v"(r1.x), "v"(r1.y), "v"(r1.z),
+                             "v"(r1.w), "v"(r2.x), "v"(r2.y), "v"(r2.z), "v"(r2.w));
+                // Live in-box lanes as a wave mask (the per-lane conditions below
+                // are all wave masks, combined on the scalar unit).
+                uint64_t act_mask = lm;
+                if (!((cover >> kb) & 1ull)) {  // wave-uniform
+                    const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
+                    act_mask &= lane_mask(px >= bx0) & lane_mask(px < bx1) & lane_mask(py >= by0) & lane_mask(py < by1);
+                }
                 const float cx0 = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
                 const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w);
                 const float nc = __uint_as_float(r1.x);
@@ -140,7 +152,8 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
                 if ((act_mask & lane_mask(pw >= __uint_as_float(r2.w))) == 0ull) continue;
                 const float op = __uint_as_float(r1.y);
                 const float alpha = fminf(exp_neg_times(pw, op), ALPHA_MAX);
-                if (act && pw <= 0.0f && alpha >= ALPHA_MIN) {
+                const uint64_t upd = act_mask & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
                     const float wgt = alpha * T;
                     cr = cr + __uint_as_float(r1.z) * wgt;
                     cg = cg + __uint_as_float(r1.w) * wgt;

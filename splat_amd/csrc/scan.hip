@@ -99,6 +99,10 @@ constexpr int SCAN_MULTI = 3;
 struct ScanArrays {
     uint32_t *data[SCAN_MULTI];
     uint32_t *row_total[SCAN_MULTI];
+    // With one row per array: the row total goes here and row_total[k][0]
+    // becomes 0, the exclusive scan of that single total (a caller that would
+    // scan the row totals next needs no second launch).
+    uint32_t *single_total[SCAN_MULTI];
 };
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, uint64_t row_len, uint32_t rows) {
     const uint32_t k = blockIdx.x / rows, r = blockIdx.x % rows;
@@ -112,7 +116,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, 
     }
     if (a.row_total[k] && threadIdx.x == 0) {
         const double wraps = rint(((double)fsum - (double)carry) * (1.0 / 4294967296.0));
-        a.row_total[k][r] = (fsum >= 68719476736.0f || wraps >= 1.0) ? 0xFFFFFFFFu : carry;
+        const uint32_t tot = (fsum >= 68719476736.0f || wraps >= 1.0) ? 0xFFFFFFFFu : carry;
+        if (a.single_total[k]) {
+            *a.single_total[k] = tot;
+            a.row_total[k][r] = 0u;
+        } else {
+            a.row_total[k][r] = tot;
+        }
     }
 }
 
@@ -156,13 +166,15 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 }
 
 hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
-                                  uint64_t row_len, uint32_t rows) {
+                                  uint64_t row_len, uint32_t rows, uint32_t *const *single_total) {
     if (rows == 0 || narrays <= 0) return hipSuccess;
     if (narrays > SCAN_MULTI) return hipErrorInvalidValue;
+    if (single_total && rows != 1) return hipErrorInvalidValue;
     ScanArrays a{};
     for (int k = 0; k < narrays; ++k) {
         a.data[k] = data[k];
         a.row_total[k] = row_total[k];
+        a.single_total[k] = single_total && row_total[k] ? single_total[k] : nullptr;
     }
     hipLaunchKernelGGL(k_scan_rows_multi, dim3(rows * (uint32_t)narrays), dim3(SCAN_THREADS), 0, s, a, row_len, rows);
     return hipGetLastError();

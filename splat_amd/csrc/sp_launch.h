@@ -14,9 +14,11 @@ namespace sp {
 // row); row_base[r] (optional) is added; row_total[r] (optional) gets the sum.
 hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t rows,
                             const uint32_t *row_base, uint32_t *row_total);
-// The same for up to 3 arrays of `rows` rows each, in one launch.
+// The same for up to 3 arrays of `rows` rows each, in one launch. With
+// single_total (rows == 1 only), array k's total is written to
+// *single_total[k] (where non-null) and its row total entry set to 0.
 hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
-                                  uint64_t row_len, uint32_t rows);
+                                  uint64_t row_len, uint32_t rows, uint32_t *const *single_total = nullptr);
 // Radix tables: rows (segment g, digit d < bins); only the live prefix of each row
 // (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries, n_cap when
 // seg_counts is null) is scanned, in place and from 0; row_total[g * bins + d]
@@ -36,6 +38,18 @@ uint32_t project_blocks(uint32_t n);
 // entries) get view 0's full per-splat projection.
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
                           uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
+
+// ---- prep.hip: scene layout on the device ----
+// The caller's arrays of structures, on the device: positions [n][3], cov3d
+// [n][6] (f32 storage) or scales [n][3] and unit quaternions [n][4] (16-bit
+// storage), opacities [n], k [n], base colours [n][3], SH rest [n][rest][3].
+struct PrepInput {
+    const float *pos, *cov, *scl, *quat, *opac, *k, *col, *sh;
+};
+// Writes the structure-of-arrays scene (column stride S >= n) in `storage`'s
+// layout: soa (f32 columns) and, for 16-bit storage, soa16.
+hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
+                       float *soa, uint16_t *soa16);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
 // Words of the per-block digit-count table for nseg segments of up to n_cap

@@ -768,9 +768,15 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
     // superblock pairs); then the view bases and the totals.
     uint32_t *rows_data[3] = {bcnt, bcnt_all, bsb}, *rows_tot[3] = {vtot, view_pairs, vsb};
     const int nrow_arrays = PHASE == EMIT_SECOND ? 1 : (bin ? 3 : 2);
-    if (e == hipSuccess) e = launch_scan_rows_multi(s, nrow_arrays, rows_data, rows_tot, bpv, nviews);
-    uint32_t *view_data[2] = {vtot, vsb}, *view_tot[2] = {counters + count_slot, counters + CNT_SBPAIRS};
-    if (e == hipSuccess) e = launch_scan_rows_multi(s, bin ? 2 : 1, view_data, view_tot, nviews, 1);
+    if (nviews == 1) {
+        // One view: its base is 0 and its totals are the frame's, written by the same launch.
+        uint32_t *single[3] = {counters + count_slot, nullptr, counters + CNT_SBPAIRS};
+        if (e == hipSuccess) e = launch_scan_rows_multi(s, nrow_arrays, rows_data, rows_tot, bpv, 1, single);
+    } else {
+        if (e == hipSuccess) e = launch_scan_rows_multi(s, nrow_arrays, rows_data, rows_tot, bpv, nviews);
+        uint32_t *view_data[2] = {vtot, vsb}, *view_tot[2] = {counters + count_slot, counters + CNT_SBPAIRS};
+        if (e == hipSuccess) e = launch_scan_rows_multi(s, bin ? 2 : 1, view_data, view_tot, nviews, 1);
+    }
     if (e != hipSuccess) return e;
     if (bin) {
         const uint64_t chunks = (pair_cap + EXPAND_CHUNK_MIN - 1) / EXPAND_CHUNK_MIN;
