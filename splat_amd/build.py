@@ -25,7 +25,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 
-HIP_SOURCES = ["scan.hip", "project.hip", "radix.hip", "tiles.hip", "blend.hip", "diff.hip", "term.hip", "api.hip"]
+HIP_SOURCES = ["scan.hip", "prep.hip", "project.hip", "radix.hip", "tiles.hip", "blend.hip", "diff.hip", "term.hip", "api.hip"]
 HOST_SOURCES = ["scene_host.cpp", "scene_io.cpp"]
 ARCH = os.environ.get("SPLAT_HIP_ARCH", "gfx950")
 

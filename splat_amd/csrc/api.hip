@@ -24,6 +24,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "half_conv.h"
@@ -772,54 +773,74 @@ int render_one(sp_handle *h, const sp_view *view, uint32_t width, uint32_t heigh
     return run_views(h, 1, view, wh, bg, outs, d_out != nullptr ? BL_RGB : BL_NONE, !h->opt.timing, stats);
 }
 
-// Upload the scene in the handle's storage format.
+// Upload the scene in the handle's storage format. The caller's arrays go to
+// the device as they are, one copy each, and a kernel builds the
+// structure-of-arrays layout (launch_prep); k is computed on host threads
+// meanwhile, with the host's logf, as Prep::new does (src/render/cpu.rs:384-388).
 int upload_scene(sp_handle *h, const sp_scene *scene) {
     const uint32_t n = h->n, rest = h->rest;
     const uint64_t S = h->stride;
-    if (h->opt.storage == SP_STORE_F32) {
-        const uint64_t ncols = (uint64_t)sp::A_SH + 3ull * rest;
-        std::vector<float> soa(ncols * S, 0.0f);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (int c = 0; c < 3; ++c) soa[(sp::A_POS + c) * S + i] = scene->positions[3ull * i + c];
-            for (int c = 0; c < 6; ++c) soa[(sp::A_COV + c) * S + i] = scene->cov3d[6ull * i + c];
-            const float o = scene->opacities[i];
-            soa[sp::A_OPAC * S + i] = o;
-            soa[sp::A_K * S + i] = k_of(o);
-            for (int c = 0; c < 3; ++c) soa[(sp::A_COL + c) * S + i] = scene->colors[3ull * i + c];
-            for (uint32_t k = 0; k < 3 * rest; ++k)
-                soa[(sp::A_SH + k) * S + i] = scene->sh_rest[(uint64_t)3 * rest * i + k];
-        }
-        h->soa_bytes = soa.size() * 4;
-        int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
-        if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(h->d_soa, soa.data(), h->soa_bytes, hipMemcpyHostToDevice, h->stream));
+    const bool f32 = h->opt.storage == SP_STORE_F32;
+    h->soa_bytes = (f32 ? (uint64_t)sp::A_SH + 3ull * rest : 5ull) * S * 4;
+    h->soa16_bytes = f32 ? 0 : ((uint64_t)sp::H_SH + 3ull * rest) * S * 2;
+    int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
+    if (!rc && !f32) rc = dalloc(h, (void **)&h->d_soa16, h->soa16_bytes);
+    if (rc) return rc;
+    // Columns are padded to S: the padding reads as zero.
+    HIP_TRY(hipMemsetAsync(h->d_soa, 0, h->soa_bytes, h->stream));
+    if (!f32) HIP_TRY(hipMemsetAsync(h->d_soa16, 0, h->soa16_bytes, h->stream));
+    if (n == 0) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         return SP_OK;
     }
-    // 16-bit storage: f32 position, opacity, k; 16-bit scale, quaternion, colour, SH.
-    uint16_t (*cvt)(float) = h->opt.storage == SP_STORE_F16 ? sp::f32_to_f16 : sp::f32_to_bf16;
-    std::vector<float> soa(5 * S, 0.0f);
-    const uint64_t hcols = (uint64_t)sp::H_SH + 3ull * rest;
-    std::vector<uint16_t> soa16(hcols * S, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        for (int c = 0; c < 3; ++c) soa[(sp::L_POS + c) * S + i] = scene->positions[3ull * i + c];
-        const float o = scene->opacities[i];
-        soa[sp::L_OPAC * S + i] = o;
-        soa[sp::L_K * S + i] = k_of(o);
-        for (int c = 0; c < 3; ++c) soa16[(sp::H_SCALE + c) * S + i] = cvt(scene->scales[3ull * i + c]);
-        for (int c = 0; c < 4; ++c) soa16[(sp::H_QUAT + c) * S + i] = cvt(scene->rotations[4ull * i + c]);
-        for (int c = 0; c < 3; ++c) soa16[(sp::H_COL + c) * S + i] = cvt(scene->colors[3ull * i + c]);
-        for (uint32_t k = 0; k < 3 * rest; ++k)
-            soa16[(sp::H_SH + k) * S + i] = cvt(scene->sh_rest[(uint64_t)3 * rest * i + k]);
+    // k on host threads while the copies run.
+    std::vector<float> kv(n);
+    const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint32_t nt = std::max(1u, std::min(hw, n / 65536u));
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            const uint64_t b = (uint64_t)n * t / nt, e = (uint64_t)n * (t + 1) / nt;
+            for (uint64_t i = b; i < e; ++i) kv[i] = k_of(scene->opacities[i]);
+        });
+    // Staging: the caller's arrays back to back.
+    const uint64_t geo = f32 ? 6ull : 7ull;  // cov3d, or scale + quaternion
+    const uint64_t per = 3 + geo + 1 + 1 + 3 + 3ull * rest;
+    float *tmp = nullptr;
+    rc = dalloc(h, (void **)&tmp, per * n * 4);
+    if (rc) {
+        for (auto &th : pool) th.join();
+        return rc;
     }
-    h->soa_bytes = soa.size() * 4;
-    h->soa16_bytes = soa16.size() * 2;
-    int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
-    if (!rc) rc = dalloc(h, (void **)&h->d_soa16, h->soa16_bytes);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(h->d_soa, soa.data(), h->soa_bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->d_soa16, soa16.data(), h->soa16_bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    sp::PrepInput in{};
+    float *p = tmp;
+    auto put = [&](const float *src, uint64_t count) -> const float * {
+        float *dst = p;
+        p += count;
+        if (src && count && hipMemcpyAsync(dst, src, count * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            rc = SP_E_DEVICE;
+        return dst;
+    };
+    in.pos = put(scene->positions, 3ull * n);
+    if (f32) {
+        in.cov = put(scene->cov3d, 6ull * n);
+    } else {
+        in.scl = put(scene->scales, 3ull * n);
+        in.quat = put(scene->rotations, 4ull * n);
+    }
+    in.opac = put(scene->opacities, n);
+    float *dk = p;
+    in.k = put(nullptr, n);
+    in.col = put(scene->colors, 3ull * n);
+    in.sh = put(rest ? scene->sh_rest : nullptr, 3ull * rest * n);
+    for (auto &th : pool) th.join();
+    if (!rc && hipMemcpyAsync(dk, kv.data(), 4ull * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = SP_E_DEVICE;
+    hipError_t e = rc ? hipErrorUnknown
+                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, S, h->d_soa, h->d_soa16);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree(h, tmp, per * n * 4);
+    if (rc || e != hipSuccess) return fail(SP_E_DEVICE, std::string("scene upload failed: ") + hipGetErrorString(e));
     return SP_OK;
 }
 

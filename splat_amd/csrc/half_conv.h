@@ -5,11 +5,19 @@
 #include <stdint.h>
 #include <string.h>
 
+#if defined(__HIPCC__)
+#define SP_HALF_HD __host__ __device__
+#else
+#define SP_HALF_HD
+#endif
+
 namespace sp {
 
-inline uint16_t f32_to_f16(float f) {
+// The same code converts on the host (tests, rounded-scene references) and on
+// the device (the scene upload), so both give the same bits.
+SP_HALF_HD inline uint16_t f32_to_f16(float f) {
     uint32_t x;
-    memcpy(&x, &f, 4);
+    __builtin_memcpy(&x, &f, 4);
     const uint32_t sign = (x >> 16) & 0x8000u;
     const uint32_t a = x & 0x7FFFFFFFu;
     if (a >= 0x7F800000u) return (uint16_t)(sign | (a > 0x7F800000u ? 0x7E00u : 0x7C00u));  // NaN / inf
@@ -29,9 +37,9 @@ inline uint16_t f32_to_f16(float f) {
     return (uint16_t)(sign | (r >> 13));
 }
 
-inline uint16_t f32_to_bf16(float f) {
+SP_HALF_HD inline uint16_t f32_to_bf16(float f) {
     uint32_t x;
-    memcpy(&x, &f, 4);
+    __builtin_memcpy(&x, &f, 4);
     if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((x >> 16) | 0x40u);  // quiet NaN
     x += 0x7FFFu + ((x >> 16) & 1u);
     return (uint16_t)(x >> 16);

@@ -1,0 +1,81 @@
+// Scene preparation on the device (the upload half of Prep::new,
+// src/render/cpu.rs:349-393): the caller's arrays-of-structures, copied to the
+// device as they are, become the renderer's structure-of-arrays layout in one
+// kernel, one thread per splat. Each output column is written by consecutive
+// threads (coalesced); each splat's own fields are read from its contiguous
+// record.
+//
+// f32 storage: position x3, cov3d x6, opacity, k, base colour x3, SH rest.
+// 16-bit storage: f32 position x3, opacity, k; scale x3, quaternion x4, base
+// colour x3 and SH rest rounded to FP16 or BF16 (round to nearest even, the
+// host's conversion code compiled for the device, so the bits are the host's).
+// k = sqrt(2 ln(o / ALPHA_MIN)) arrives precomputed by the host, whose logf
+// the reference's bits follow.
+#include "half_conv.h"
+#include "sp_launch.h"
+
+namespace sp {
+namespace {
+
+__global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos, const float *__restrict__ cov,
+                                                  const float *__restrict__ opac, const float *__restrict__ kk,
+                                                  const float *__restrict__ col, const float *__restrict__ sh,
+                                                  uint32_t n, uint32_t rest, uint64_t S, float *__restrict__ soa) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) soa[(A_POS + c) * S + i] = pos[3ull * i + c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) soa[(A_COV + c) * S + i] = cov[6ull * i + c];
+    soa[A_OPAC * S + i] = opac[i];
+    soa[A_K * S + i] = kk[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) soa[(A_COL + c) * S + i] = col[3ull * i + c];
+    for (uint32_t c = 0; c < 3 * rest; ++c) soa[(A_SH + c) * S + i] = sh[(uint64_t)3 * rest * i + c];
+}
+
+template <bool BF16>
+__device__ __forceinline__ uint16_t to16(float f) {
+    return BF16 ? f32_to_bf16(f) : f32_to_f16(f);
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, const float *__restrict__ scl,
+                                                 const float *__restrict__ quat, const float *__restrict__ opac,
+                                                 const float *__restrict__ kk, const float *__restrict__ col,
+                                                 const float *__restrict__ sh, uint32_t n, uint32_t rest, uint64_t S,
+                                                 float *__restrict__ soa, uint16_t *__restrict__ soa16) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) soa[(L_POS + c) * S + i] = pos[3ull * i + c];
+    soa[L_OPAC * S + i] = opac[i];
+    soa[L_K * S + i] = kk[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) soa16[(H_SCALE + c) * S + i] = to16<BF16>(scl[3ull * i + c]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) soa16[(H_QUAT + c) * S + i] = to16<BF16>(quat[4ull * i + c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) soa16[(H_COL + c) * S + i] = to16<BF16>(col[3ull * i + c]);
+    for (uint32_t c = 0; c < 3 * rest; ++c) soa16[(H_SH + c) * S + i] = to16<BF16>(sh[(uint64_t)3 * rest * i + c]);
+}
+
+}  // namespace
+
+hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
+                       float *soa, uint16_t *soa16) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (n + 255) / 256;
+    if (storage == SP_STORE_F32)
+        hipLaunchKernelGGL(k_prep_f32, dim3(grid), dim3(256), 0, s, in.pos, in.cov, in.opac, in.k, in.col, in.sh, n,
+                           rest, S, soa);
+    else if (storage == SP_STORE_F16)
+        hipLaunchKernelGGL(k_prep_16<false>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
+                           in.col, in.sh, n, rest, S, soa, soa16);
+    else
+        hipLaunchKernelGGL(k_prep_16<true>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
+                           in.col, in.sh, n, rest, S, soa, soa16);
+    return hipGetLastError();
+}
+
+}  // namespace sp

@@ -19,10 +19,11 @@ hipError_t launch_scan_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uin
 // *single_total[k] (where non-null) and its row total entry set to 0.
 hipError_t launch_scan_rows_multi(hipStream_t s, int narrays, uint32_t *const *data, uint32_t *const *row_total,
                                   uint64_t row_len, uint32_t rows, uint32_t *const *single_total = nullptr);
-// Radix tables: rows (segment g, digit d < bins); only the live prefix of each row
-// (radix_columns(min(seg_counts[g], n_cap), tile_keys) entries, n_cap when
-// seg_counts is null) is scanned, in place and from 0; row_total[g * bins + d]
-// receives the row's sum.
+// Radix tables: per segment g, row_len columns of bins digit counts each
+// (entry (g, c, d) at data[(g * row_len + c) * bins + d]); for every (g, d)
+// only the live columns (radix_columns(min(seg_counts[g], n_cap), tile_keys),
+// n_cap when seg_counts is null) are scanned across the columns, in place and
+// from 0; row_total[g * bins + d] receives the sum.
 hipError_t launch_scan_digit_rows(hipStream_t s, uint32_t *data, uint64_t row_len, uint32_t nseg, uint32_t bins,
                                   const uint32_t *seg_counts, uint64_t n_cap, uint32_t tile_keys,
                                   uint32_t *row_total);
