@@ -13,8 +13,11 @@ Also reported beside the headline value:
     one sp_render at a time,
   * cpu_baseline: the oracle (C++ restatement of splat-tui's `cpu` backend) on a
     bounded sample of the same views, timed on this host's cores.
-  * precision: 3,000,000 splats stored as f32 / FP16 / BF16: views/s and each
-    stage's error against the FP64 oracle (conics, colours, final image).
+  * precision: 3,000,000 splats stored as f32 / FP16 / BF16: views/s, upload
+    time and device bytes (each stage's error against the FP64 oracle is
+    checked at this size by tests/test_gpu_configs.py).
+  * long_lists: the 1M scene with low opacities (sigmoid(N(-3, 1.5))), whose
+    tiles saturate late: views/s, blend time, longest tile list.
   * scene files: the same scene written as a 3DGS PLY and as .splat, read back
     by the native loader (parse + finalize) and uploaded; load rate per thread
     count and the render of the loaded scene against the original.
@@ -23,8 +26,12 @@ Also reported beside the headline value:
     hip backend at f32, FP16 and BF16 storage, 1920x1080, 40 frames.
   * differentiation: 500,000 splats, 8 views at 1280x720, L2 loss against
     targets rendered from parameters perturbed by N(0, 0.01): forward and
-    backward wall times, and the gradient checked against the FP64 reference
-    gradient and FP64 central differences on 256 random parameters.
+    backward wall times (the gradient is checked against the FP64 reference
+    gradient and FP64 central differences at this size by
+    tests/test_gpu_configs.py).
+
+Only the cpu_baseline leg runs the oracle (as the CPU baseline and to check
+the view it times byte for byte); every other figure is the HIP path's.
 
 python bench.py [--gpus 1] [--steps 40] [--warmup 5]
 """
@@ -76,8 +83,8 @@ def main() -> None:
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-splats", type=int, default=3_000_000)
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-long-lists", action="store_true", help="skip the low-opacity long-list leg")
     ap.add_argument("--diff-splats", type=int, default=500_000)
-    ap.add_argument("--gradcheck-params", type=int, default=256)
     ap.add_argument("--no-io", action="store_true", help="skip the scene-file leg")
     ap.add_argument("--no-orbit", action="store_true", help="skip the reference-format orbit table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (profiling runs)")
@@ -268,6 +275,11 @@ def main() -> None:
             result["precision"] = bench_precision(S, torch, args, W, H)
         except Exception as e:
             result["precision"] = {"error": repr(e)}
+    if not args.no_long_lists:
+        try:
+            result["long_lists"] = bench_long_lists(S, torch, W, H, min(args.steps, 20))
+        except Exception as e:
+            result["long_lists"] = {"error": repr(e)}
     print(json.dumps(result))
 
 
@@ -284,12 +296,8 @@ def host_cores() -> int:
 
 def bench_diff(S, torch, args) -> dict:
     """Forward + backward of an L2 image loss for 500k SH-3 splats over 8 views at
-    1280x720 (wall times around synchronous calls, median of 5), then the GPU
-    gradient against the FP64 reference: relative L2 error per parameter group and
-    central differences (step 1e-6) on random parameters of visible splats."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    from splat_amd.diff import PARAM_KEYS, DiffRenderer, params_from_scene
+    1280x720: wall times around synchronous calls, median of 5 after a warm-up."""
+    from splat_amd.diff import DiffRenderer, params_from_scene
 
     n = args.diff_splats
     log(f"differentiation: {n} splats, 8 views 1280x720")
@@ -314,7 +322,7 @@ def bench_diff(S, torch, args) -> dict:
         gimg = [2.0 * (imgs[v] - targets[v]) for v in range(len(views))]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        g = r.backward(P, gimg)
+        r.backward(P, gimg)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
         if it:  # first iteration warms allocations
@@ -322,7 +330,7 @@ def bench_diff(S, torch, args) -> dict:
             bwd.append((t3 - t2) * 1e3)
     loss = float(sum(((imgs[v] - targets[v]) ** 2).sum().item() for v in range(len(views))))
     pairs = int(sum(st.overlaps for st in r.last_stats))
-    out = {
+    return {
         "workload": f"{n} random SH-3 splats, {len(views)} orbit views at 1280x720, L2 loss vs targets from "
                     "parameters + N(0, 0.01)",
         "forward_ms": round(float(np.median(fwd)), 3),
@@ -330,76 +338,25 @@ def bench_diff(S, torch, args) -> dict:
         "backward_over_forward": round(float(np.median(bwd)) / float(np.median(fwd)), 3),
         "pairs": pairs,
         "loss": loss,
+        "gradient_check": "tests/test_gpu_configs.py::test_gradients_500k_8x720p (FP64 reference and differences)",
     }
-    if args.gradcheck_params <= 0:
-        return out
-    from oracle import oracle as O
-
-    cores = host_cores()
-    log(f"fp64 reference gradient on {cores} threads")
-    gn = {k: g[k].cpu().numpy() for k in PARAM_KEYS}
-    G = np.concatenate([gn["means"], gn["log_scales"], gn["quats"], gn["opacity_logits"].reshape(n, 1),
-                        gn["colors"], gn["sh_rest"].reshape(n, -1)], axis=1).astype(np.float64)
-    tnp = [t.cpu().numpy() for t in targets]
-    t0 = time.perf_counter()
-    D = O.DiffF64(prm, 3, views, sizes, bg, tnp, threads=cores)
-    G64 = D.grad_flat()
-    groups = {"means": slice(0, 3), "log_scales": slice(3, 6), "quats": slice(6, 10),
-              "opacity_logits": slice(10, 11), "colors": slice(11, 14), "sh_rest": slice(14, None)}
-    rel_l2 = {k: float(np.linalg.norm(G[:, sl] - G64[:, sl]) / max(np.linalg.norm(G64[:, sl]), 1e-30))
-              for k, sl in groups.items()}
-    rows = np.nonzero(np.abs(G64).sum(1))[0]
-    prng = np.random.default_rng(1)
-    picks = [(int(prng.choice(rows)), int(prng.integers(0, D.np))) for _ in range(args.gradcheck_params)]
-    # Central differences at two steps. The loss is only piecewise smooth (alpha
-    # crosses 1/255, T crosses 1e-4, boxes move by a pixel); where the two steps
-    # disagree, the stencil straddles such a jump and no difference quotient
-    # estimates the derivative. Errors are relative with a 1e-6 floor (the
-    # difference quotient's round-off level for this loss).
-    with ThreadPoolExecutor(max_workers=cores) as ex:
-        fd5 = list(ex.map(lambda ik: D.fd(ik[0], ik[1], 1e-5), picks))
-        fd6 = list(ex.map(lambda ik: D.fd(ik[0], ik[1], 1e-6), picks))
-    smooth = np.array([abs(a - b) <= 1e-3 * max(abs(a), abs(b)) + 1e-7 for a, b in zip(fd5, fd6)])
-    rel = np.array([abs(G[i, k] - fd) / max(abs(fd), abs(G[i, k]), 1e-6) for (i, k), fd in zip(picks, fd5)])
-    rs = rel[smooth] if smooth.any() else np.zeros(1)
-    out["gradient_check"] = {
-        "fp64_loss": D.loss(),
-        "rel_l2_vs_fp64_gradient": {k: float(f"{v:.3e}") for k, v in rel_l2.items()},
-        "fd_params": len(picks),
-        "fd_step": 1e-5,
-        "fd_rel_err_median": float(f"{np.median(rel):.3e}"),
-        "fd_rel_err_p95": float(f"{np.percentile(rel, 95):.3e}"),
-        "fd_rel_err_max": float(f"{rel.max():.3e}"),
-        "fd_smooth_params": int(smooth.sum()),
-        "fd_smooth_rel_err_median": float(f"{np.median(rs):.3e}"),
-        "fd_smooth_rel_err_max": float(f"{rs.max():.3e}"),
-        "fp64_seconds": round(time.perf_counter() - t0, 1),
-        "cores": cores,
-    }
-    return out
 
 
 def bench_precision(S, torch, args, W, H) -> dict:
     """3M SH-3 splats with the covariance factors (scale, rotation), colours and SH
-    stored as f32 / FP16 / BF16: views/s over 20 orbit views, and for one view the
-    error of each stage against the FP64 oracle (projection of every splat, full
-    FP64 render)."""
-    from oracle import oracle as O
-
+    stored as f32 / FP16 / BF16: scene upload time, device bytes and views/s over
+    20 orbit views."""
     n = args.precision_splats
     log(f"precision: {n} splats")
     scene = S.random_scene(n, seed=1, sh_degree=3)
     views = S.orbit_views(scene, 40, 20.0)
-    v0 = views[0]
-    cores = host_cores()
-    v64, p64 = O.project_f64(scene, v0, W, H, threads=cores)
-    rgb64, img64 = O.render_f64(scene, v0, W, H, (0, 0, 0), threads=cores)
-    out = {"workload": f"{n} random SH-3 splats, {W}x{H}, 40-frame orbit", "storages": {}}
+    out = {"workload": f"{n} random SH-3 splats, {W}x{H}, 40-frame orbit", "storages": {},
+           "errors_vs_fp64": "tests/test_gpu_configs.py::test_precision_3m_1080p"}
     dev = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
     for storage in ("f32", "f16", "bf16"):
         torch.cuda.synchronize()
         t_up = time.perf_counter()
-        r = S.HipRenderer(scene, storage=storage)
+        r = S.HipRenderer(scene, storage=storage, timing=False)
         upload_ms = (time.perf_counter() - t_up) * 1e3
         for i in range(3):
             r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
@@ -410,33 +367,54 @@ def bench_precision(S, torch, args, W, H) -> dict:
             r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
         torch.cuda.synchronize()
         vps = k / (time.perf_counter() - t0)
-        valid, p, _ = r.debug_project(v0, W, H)
-        both = valid & v64
-        conic = p[both][:, 2:5].astype(np.float64)
-        c64 = p64[both][:, 2:5]
-        crel = np.abs(conic - c64) / np.abs(c64).max(axis=1, keepdims=True)
-        col = np.abs(p[both][:, 5:8].astype(np.float64) - p64[both][:, 5:8])
-        img, _ = r.render(v0, W, H, (0, 0, 0))
-        d = np.abs(img.pixels.astype(int) - img64.astype(int))
         out["storages"][storage] = {
             "views_per_s": round(vps, 3),
             "scene_upload_ms": round(upload_ms, 2),
             "device_bytes": r.frame_info()["device_bytes"],
-            "cull_mismatch_vs_fp64": int((valid != v64).sum()),
-            "conic_rel_err_max": float(f"{crel.max():.3e}"), "conic_rel_err_mean": float(f"{crel.mean():.3e}"),
-            "color_abs_err_max": float(f"{col.max():.3e}"), "color_abs_err_mean": float(f"{col.mean():.3e}"),
-            "image_psnr_vs_fp64": round(float(S.psnr(img.pixels, img64)), 3),
-            "image_max_abs_err_vs_fp64": int(d.max()),
         }
-        if storage != "f32":
-            # The renderer's own error: against an FP64 render of the parameters the
-            # 16-bit storage actually holds.
-            _, img_r = O.render_f64(scene.storage_rounded(storage), v0, W, H, (0, 0, 0), threads=cores)
-            dr = np.abs(img.pixels.astype(int) - img_r.astype(int))
-            out["storages"][storage]["image_psnr_vs_fp64_of_rounded_scene"] = round(float(S.psnr(img.pixels, img_r)), 3)
-            out["storages"][storage]["image_max_abs_err_vs_fp64_of_rounded_scene"] = int(dr.max())
         r.close()
     return out
+
+
+def bench_long_lists(S, torch, W, H, steps: int) -> dict:
+    """The 1M SH-3 scene with opacity = sigmoid(N(-3, 1.5)): tiles saturate late, so
+    tile lists are long and the blend walks far more of them. views/s (graph
+    replays), the blend's share (a timed handle) and the longest tile list."""
+    scene = S.random_scene(1_000_000, seed=1, sh_degree=3)
+    rng = np.random.default_rng(7)
+    logits = rng.normal(-3.0, 1.5, len(scene)).astype(np.float32)
+    scene.opacities = (1.0 / (1.0 + np.exp(-logits.astype(np.float64)))).astype(np.float32)
+    scene.finalize()
+    views = S.orbit_views(scene, 40, 20.0)
+    out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
+    r = S.HipRenderer(scene, timing=False)
+    tuned = r.autotune(views[:1], [(W, H)], (0, 0, 0), reps=3)
+    for i in range(3):
+        r.render_device(views[i], W, H, (0, 0, 0), out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.render_device(views[i % len(views)], W, H, (0, 0, 0), out.data_ptr())
+    torch.cuda.synchronize()
+    vps = steps / (time.perf_counter() - t0)
+    rt = S.HipRenderer(scene)
+    rt.set_tuning(**{k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")})
+    rt.render_device(views[0], W, H, (0, 0, 0), out.data_ptr())
+    st = [rt.render_device(views[i], W, H, (0, 0, 0), out.data_ptr()) for i in range(4)]
+    info = rt.frame_info()
+    _, _, ts = rt.debug_pairs(views[3], W, H)
+    lens = np.diff(ts.astype(np.int64))
+    rt.close()
+    r.close()
+    return {
+        "workload": "1M random SH-3 splats with opacity = sigmoid(N(-3, 1.5)), 1920x1080, 40-frame orbit",
+        "views_per_s": round(vps, 3),
+        "blend_ms": round(float(np.mean([s.raster_ms for s in st])), 4),
+        "project_sort_emit_ms": round(float(np.mean([s.project_ms for s in st])), 4),
+        "pairs": info["pairs"], "pairs_staged_by_blend": info["pairs_staged"],
+        "max_tile_list": int(lens.max()), "mean_tile_list": round(float(lens.mean()), 1),
+        "tuning": {k: tuned[k] for k in ("blend_batch", "split_shift", "split_kmin")},
+    }
 
 
 def bench_batched(S, torch, scene, nviews: int, autotune: bool = True) -> dict:

@@ -5,9 +5,9 @@
 // pass, e.g. a 9-bit sort in one pass instead of 8 + 1):
 //   * upsweep: each block counts the digits of a tile (4096 keys, or 1024 for
 //     sorts small enough that 4096-key tiles would leave CUs idle) into a
-//     (segment, digit, column) table, a column being 1-8 consecutive tiles,
+//     (segment, column, digit) table, a column being 1-8 consecutive tiles,
 //   * digit-row scan: one workgroup per (segment, digit) row turns the counts
-//     into in-row positions and the row's total,
+//     of that digit across the columns into positions and the row's total,
 //   * downsweep: each block scans its segment's row totals into digit
 //     bases (so no histogram pass is needed), re-reads its tile, ranks it with a wave64 ballot
 //     multisplit (8 ballots give the mask of lanes sharing the digit; the rank
@@ -42,10 +42,12 @@ namespace {
 constexpr int NWAVES = RADIX_THREADS / 64;
 constexpr uint32_t RADIX_GRID = 4096;  // bound on the upsweep / downsweep grid
 
-// table[(seg * BINS + d) * bps + c] = count of digit d in table column c of
-// `seg` (radix_group consecutive tiles of 64 * NWAVES * ITEMS keys). gps workgroups per segment
-// stride over the segment's live columns, so the grid follows the capacity
-// only up to a bound and idle work follows the live count.
+// table[(seg * bps + c) * BINS + d] = count of digit d in table column c of
+// `seg` (radix_group consecutive tiles of 64 * NWAVES * ITEMS keys): a
+// column's counts are contiguous, so each workgroup writes (and the downsweep
+// reads) one block of BINS words. gps workgroups per segment stride over the
+// segment's live columns, so the grid follows the capacity only up to a bound
+// and idle work follows the live count.
 template <int ITEMS, int BINS, bool SPARSE>
 __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ seg_counts,
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_upsweep(const uint32_t *__res
             uint32_t c = 0;
 #pragma unroll
             for (int ww = 0; ww < NWAVES; ++ww) c += s_h[ww][d];
-            table[((uint64_t)seg * BINS + d) * bps + col] = c;
+            table[((uint64_t)seg * bps + col) * BINS + d] = c;
         }
         __syncthreads();
     }
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_downsweep(const uint32_t *__r
         uint32_t run[DPT];
 #pragma unroll
         for (int j = 0; j < DPT; ++j)
-            run[j] = row_base[j] + table[((uint64_t)seg * BINS + threadIdx.x * DPT + j) * bps + col];
+            run[j] = row_base[j] + table[((uint64_t)seg * bps + col) * BINS + threadIdx.x * DPT + j];
         const uint32_t t_end = min(tiles, (col + 1) * group);
         for (uint32_t t = col * group; t < t_end; ++t) {
             downsweep_tile<ITEMS, BINS, SPARSE>(ks, vs, ko, vo, n, t, shift, width, run, s_wh, s_lstart, s_gbase,

@@ -126,24 +126,48 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_rows_multi(ScanArrays a, 
     }
 }
 
-// Radix digit rows: row r = (segment g, digit d) of `row_len` per-column
-// counts, of which the first radix_columns(min(seg_counts[g], n_cap),
-// tile_keys) are live.
-// Each live prefix becomes its in-row exclusive scan and row_total[r] its sum
-// (the sort's downsweep turns the row totals of a segment into digit
-// bases itself, so no separate histogram pass is needed).
+// Radix digit rows: row (segment g, digit d) holds the digit's count in each
+// of the segment's columns, at data[(g * row_len + c) * bins + d] (a column's
+// counts are contiguous), of which the first radix_columns(min(seg_counts[g],
+// n_cap), tile_keys) are live. Each live prefix becomes its exclusive scan and
+// row_total[g * bins + d] its sum (the sort's downsweep turns the row totals of
+// a segment into digit bases itself, so no separate histogram pass is
+// needed).
+// A 128-byte line of a column holds 32 digits. Workgroups are dealt to the 8
+// XCDs round-robin, so block b takes a digit on XCD b % 8's share of the
+// digits (bins / 8 consecutive ones): each line is then read and written
+// through one XCD's L2 only.
+constexpr uint32_t XCDS = 8;
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_digit_rows(uint32_t *__restrict__ data, uint64_t row_len,
                                                                   uint32_t bins,
                                                                   const uint32_t *__restrict__ seg_counts,
                                                                   uint64_t n_cap, uint32_t tile_keys,
                                                                   uint32_t *__restrict__ row_total) {
-    const uint32_t g = blockIdx.x / bins;
+    const uint32_t per = bins / XCDS, x = blockIdx.x % XCDS, j = blockIdx.x / XCDS;
+    const uint32_t g = j / per, d = x * per + j % per;
     const uint64_t n = seg_counts ? min((uint64_t)seg_counts[g], n_cap) : n_cap;  // none: every segment holds n_cap
     const uint64_t live = min(row_len, (uint64_t)radix_columns(n, tile_keys));
-    uint32_t *row = data + (uint64_t)blockIdx.x * row_len;
+    uint32_t *col0 = data + (uint64_t)g * row_len * bins + d;
     uint32_t carry = 0;
-    for (uint64_t off = 0; off < live; off += SCAN_CHUNK) carry += scan_chunk(row, off, live, carry, true);
-    if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
+    for (uint64_t off = 0; off < live; off += SCAN_CHUNK) {
+        uint32_t v[SCAN_ITEMS];
+        uint32_t sum = 0;
+        const uint64_t b = off + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            v[k] = (b + k < live) ? col0[(b + k) * bins] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t pre = block_exclusive_scan<SCAN_THREADS>(sum, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (b + k < live) col0[(b + k) * bins] = pre;
+            pre += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) row_total[(uint64_t)g * bins + d] = carry;
 }
 
 __global__ __launch_bounds__(256) void k_fill(uint32_t *__restrict__ p, uint64_t n, uint32_t value) {
