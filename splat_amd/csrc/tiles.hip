@@ -44,6 +44,10 @@
 namespace sp {
 namespace {
 
+// Frames with at most this many rows of BIN_SCAN_ROW tiles (65536 tiles, a
+// 4096x4096 frame) have k_bin_write sum the row totals itself instead of a
+// scan launch.
+constexpr uint32_t BIN_ROWS_SUMMED = 32;
 // Superblock pairs per workgroup step of the expansion: the chunk follows the
 // live count (512 to 4096 pairs), so few pairs still spread over many CUs.
 constexpr uint32_t EXPAND_CHUNK_MAX = 4096;
@@ -58,25 +62,29 @@ struct Rect {
 // Tile rectangle of the j-th depth-ordered splat of view v, from the
 // depth-ordered pixel boxes (box[vbase + j] = the record's packed x0|y0, x1|y1);
 // with WITH_ID, also its record index (the count passes do not need it).
+// Tile rectangle of a packed pixel box (x0 | y0 << 16, x1 | y1 << 16).
+__device__ __forceinline__ Rect box_rect(uint2 b, uint32_t tiles_x, uint32_t tile_base) {
+    Rect r{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
+    const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
+    if (x0 < x1 && y0 < y1) {
+        r.tx0 = x0 / TILE;
+        r.ty0 = y0 / TILE;
+        r.tx1 = (x1 + TILE - 1) / TILE;
+        r.ty1 = (y1 + TILE - 1) / TILE;
+        r.rw = r.tx1 - r.tx0;
+        r.cnt = r.rw * (r.ty1 - r.ty0);
+        r.org = tile_base + r.ty0 * tiles_x + r.tx0;
+    }
+    return r;
+}
+
 template <bool WITH_ID = true>
 __device__ __forceinline__ Rect splat_rect(const uint32_t *__restrict__ dval, const uint2 *__restrict__ box,
                                            uint64_t vbase, uint32_t j, uint32_t nvis, uint32_t tiles_x,
                                            uint32_t tile_base) {
-    Rect r{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
-    if (j < nvis) {
-        if (WITH_ID) r.id = (uint32_t)(vbase + dval[vbase + j]);
-        const uint2 b = box[vbase + j];
-        const uint32_t x0 = b.x & 0xFFFFu, y0 = b.x >> 16, x1 = b.y & 0xFFFFu, y1 = b.y >> 16;
-        if (x0 < x1 && y0 < y1) {
-            r.tx0 = x0 / TILE;
-            r.ty0 = y0 / TILE;
-            r.tx1 = (x1 + TILE - 1) / TILE;
-            r.ty1 = (y1 + TILE - 1) / TILE;
-            r.rw = r.tx1 - r.tx0;
-            r.cnt = r.rw * (r.ty1 - r.ty0);
-            r.org = tile_base + r.ty0 * tiles_x + r.tx0;
-        }
-    }
+    if (j >= nvis) return Rect{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
+    Rect r = box_rect(box[vbase + j], tiles_x, tile_base);
+    if (WITH_ID) r.id = (uint32_t)(vbase + dval[vbase + j]);
     return r;
 }
 
@@ -153,7 +161,7 @@ __device__ __forceinline__ uint32_t phase_count(const Rect &r, uint32_t j, uint3
 template <int PHASE>
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                              uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                             const uint2 *__restrict__ box,
+                                                             const uint2 *__restrict__ sbox, uint2 *__restrict__ box,
                                                              const ViewParams *__restrict__ views, EmitSplit split,
                                                              const uint32_t *__restrict__ ip,
                                                              uint32_t *__restrict__ bcnt,
@@ -171,7 +179,21 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
     const uint32_t k1 = first_count(nvis, split);
     const uint32_t tiles_x = views[v].ctx.tiles_x, tile_base = views[v].tile_base;
     const uint32_t j = local * EMIT_THREADS + threadIdx.x;
-    const Rect r = splat_rect<false>(dval, box, vbase, j, nvis, tiles_x, tile_base);
+    Rect r;
+    if (PHASE == EMIT_FIRST) {
+        // The first pass over the depth order also gathers each splat's box
+        // from the scene-order boxes the projection wrote (8 bytes per splat:
+        // a view's array stays cache-resident, where the 48-byte records would
+        // not), so that every later pass reads the boxes in depth order.
+        r = Rect{0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u};
+        if (j < nvis) {
+            const uint2 b = sbox[vbase + dval[vbase + j]];
+            box[vbase + j] = b;
+            r = box_rect(b, tiles_x, tile_base);
+        }
+    } else {
+        r = splat_rect<false>(dval, box, vbase, j, nvis, tiles_x, tile_base);
+    }
     const uint32_t c = phase_count<PHASE>(r, j, k1, ip, tile_base, tiles_x);
     if (PHASE == EMIT_SECOND) {
         const uint32_t v1[1] = {c};
@@ -191,22 +213,6 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
             if (bsb) bsb[blockIdx.x] = t3[2];
         }
     }
-}
-
-// box[v*n + j] = pixel box words of view v's j-th depth-ordered splat, from
-// the scene-order boxes the projection wrote (sbox, 8 bytes per splat: a
-// view's array stays cache-resident, where the 48-byte records would not), so
-// that the emission passes read the boxes in depth order.
-__global__ __launch_bounds__(EMIT_THREADS) void k_gather_boxes(const uint32_t *__restrict__ dval, uint32_t n_splats,
-                                                               uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                               const uint2 *__restrict__ sbox,
-                                                               uint2 *__restrict__ box) {
-    const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
-    const uint64_t vbase = (uint64_t)v * n_splats;
-    const uint32_t nvis = min(vis_count[v], n_splats);
-    const uint32_t j = local * EMIT_THREADS + threadIdx.x;
-    if (j >= nvis) return;
-    box[vbase + j] = sbox[vbase + dval[vbase + j]];
 }
 
 // Pair-balanced expansion of the first phase's superblock pairs: the
@@ -548,15 +554,18 @@ __global__ __launch_bounds__(256) void k_bin_prefix(const ViewParams *__restrict
 // capacity: an over-capacity frame is rendered again with larger buffers) and
 // the segment's entries into the tile lists: pval = record index, pkey (when
 // given) = tile id. tscan: in-row exclusive prefix of tcount (rows of
-// BIN_SCAN_ROW tiles); trow: exclusive prefix of the row totals; segc: the
-// segment offsets within each tile's list.
+// BIN_SCAN_ROW tiles); trow: the row totals, or with prefixed_rows their
+// exclusive prefix (frames of more than BIN_ROWS_SUMMED rows scan them in a
+// launch of their own; fewer are summed here); segc: the segment offsets
+// within each tile's list.
 __global__ __launch_bounds__(256) void k_bin_write(const uint32_t *__restrict__ sbtile,
                                                    const uint32_t *__restrict__ sb_start,
                                                    const uint32_t *__restrict__ skey,
                                                    const uint32_t *__restrict__ sval,
                                                    const uint32_t *__restrict__ segc,
                                                    const uint32_t *__restrict__ tscan,
-                                                   const uint32_t *__restrict__ trow, uint32_t total_tiles,
+                                                   const uint32_t *__restrict__ trow, uint32_t prefixed_rows,
+                                                   uint32_t total_tiles,
                                                    uint32_t *__restrict__ tile_start, uint32_t *__restrict__ pkey,
                                                    uint32_t *__restrict__ pval, uint64_t pair_cap,
                                                    const uint32_t *__restrict__ counters) {
@@ -581,7 +590,13 @@ __global__ __launch_bounds__(256) void k_bin_write(const uint32_t *__restrict__ 
         const uint32_t gt = sbtile[(uint64_t)sb * (SB * SB) + threadIdx.x];
         uint32_t start = 0;
         if (gt != ~0u) {
-            start = trow[gt / BIN_SCAN_ROW] + tscan[gt];
+            const uint32_t row = gt / BIN_SCAN_ROW;
+            if (prefixed_rows) {
+                start = trow[row];
+            } else {
+                for (uint32_t r = 0; r < row; ++r) start += trow[r];
+            }
+            start += tscan[gt];
             if (y == 0) tile_start[gt] = (uint32_t)min((uint64_t)start, pair_cap);
         }
         s_gt[threadIdx.x] = gt;
@@ -753,15 +768,9 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       uint32_t *powner, uint32_t *pbase, uint64_t pair_cap, uint32_t count_slot,
                       uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
-    if (PHASE == EMIT_FIRST) {
-        hipLaunchKernelGGL(k_gather_boxes, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           vis_count, sbox, box);
-        const hipError_t eg = hipGetLastError();
-        if (eg != hipSuccess) return eg;
-    }
     const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
+                       vis_count, sbox, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
     hipError_t e = hipGetLastError();
     // Per view: block offsets and the view's pair total (and, in FIRST, the
     // unsplit pair count of every view, RenderStats.overlaps, and the binned
@@ -833,10 +842,11 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
                        total_tiles, rows * BIN_SCAN_ROW);
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_scan_rows(s, tcount, BIN_SCAN_ROW, rows, nullptr, trow);
-    if (e == hipSuccess) e = launch_scan_rows(s, trow, rows, 1, nullptr, nullptr);
+    const uint32_t prefixed = rows > BIN_ROWS_SUMMED ? 1u : 0u;
+    if (e == hipSuccess && prefixed) e = launch_scan_rows(s, trow, rows, 1, nullptr, nullptr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_write, grid, dim3(256), 0, s, sbtile, sb_start, skey, sval, segc, tcount, trow,
-                       total_tiles, tile_start, pkey, pval, pair_cap, counters);
+                       prefixed, total_tiles, tile_start, pkey, pval, pair_cap, counters);
     return hipGetLastError();
 }
 
