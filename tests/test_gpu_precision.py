@@ -46,7 +46,10 @@ def test_storage_error_against_fp64(oracle, storage):
     case(f"precision {storage}")
     e = stage_errors(oracle, s, r, v, 480, 270)
     print(f"[precision] {storage}: {e}", flush=True)
-    bounds = {"f32": (2e-4, 2e-5, 50.0), "f16": (5e-3, 2e-3, 45.0), "bf16": (4e-2, 1.5e-2, 35.0)}[storage]
+    # Image bars: the storage rounding alone costs this view 66.9 dB (FP16) and
+    # 58.0 dB (BF16) against FP64 (an FP64 render of the rounded scene); the
+    # renderer's own f32 error is far smaller (see the rounded-scene check).
+    bounds = {"f32": (2e-4, 2e-5, 80.0), "f16": (5e-3, 2e-3, 64.0), "bf16": (4e-2, 1.5e-2, 56.0)}[storage]
     assert e["cull_mismatch"] <= max(5, e["visible"] // 2000)
     assert e["conic_rel_mean"] < bounds[0]
     assert e["color_abs_mean"] < bounds[1]
