@@ -52,11 +52,14 @@ def _ptr(t) -> int | None:
 class DiffRenderer:
     """A differentiable-render handle for n gaussians of SH degree sh_degree."""
 
-    def __init__(self, n: int, sh_degree: int, device: int = 0):
+    def __init__(self, n: int, sh_degree: int, device: int = 0, initial_pairs: int = 0):
+        """initial_pairs: the pair buffer's first capacity (0: automatic). A
+        forward whose pairs exceed it grows the buffer and renders again."""
         lib = N.hip()
         opt = N.SpOptions()
         lib.sp_default_options(C.byref(opt))
         opt.device = device
+        opt.initial_pairs = initial_pairs
         h = C.c_void_p()
         N.check(lib.sp_diff_create(n, sh_degree, C.byref(opt), C.byref(h)))
         self._h = h

@@ -160,3 +160,23 @@ def test_forward_image_matches_plain_renderer_within_bar(oracle):
     print(f"[diff] forward vs cpu: psnr {S.psnr(q, ref):.2f} max {d.max()}", flush=True)
     assert S.psnr(q, ref) >= 60.0
     assert d.max() <= 2
+
+
+def test_pair_capacity_growth_keeps_gradients(oracle):
+    """A differentiable handle that starts with room for 64 pairs overflows,
+    grows its pair buffer (and the per-pair owner array with it) and renders
+    again: images and gradients equal a default-capacity handle's bit for bit."""
+    case("diff capacity growth")
+    deg = 2
+    prm, views, sizes, targets, _ = problem(oracle, deg, 3000, 2, (200, 136), seed=17)
+    _, _, imgs_a, ga = gpu_run(prm, deg, views, sizes, targets)
+    n = len(prm["means"])
+    r = DiffRenderer(n, deg, initial_pairs=64)
+    P = to_dev(prm)
+    imgs = r.forward(P, views, sizes, BG)
+    assert sum(st.overlaps for st in r.last_stats) > 64
+    g = r.backward(P, [2.0 * (imgs[v] - torch.from_numpy(targets[v]).cuda()) for v in range(len(views))])
+    for v in range(len(views)):
+        assert np.array_equal(imgs[v].cpu().numpy(), imgs_a[v]), f"view {v}"
+    for k in PARAM_KEYS:
+        assert np.array_equal(g[k].cpu().numpy(), ga[k]), k
