@@ -205,8 +205,9 @@ struct sp_handle {
     uint64_t g2d_cap = 0;
     float *d_part = nullptr;  // segmented-reduction carries
     uint64_t part_cap = 0;
-    uint32_t fwd_views = 0, fwd_tiles = 0;  // shape of the last differentiable forward
+    uint32_t fwd_views = 0, fwd_tiles = 0, fwd_sbs = 0;  // shape of the last differentiable forward
     uint64_t fwd_pairs = 0;
+    uint32_t last_sbs = 0;  // superblocks of the last rendered batch
     // Buffers carved from a fixed region (sp_render_device_static): nothing may
     // grow or be freed; a frame that does not fit is SP_E_CAPACITY.
     bool fixed = false;
@@ -592,7 +593,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(mark(2));
     if (mode != BL_NONE) {
         const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
-        HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
+        HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start, h->d_pval[pi],
                                  h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast, h->d_done, h->d_pstate,
                                  h->blend_batch));
     }
@@ -616,7 +617,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     }
     HIP_TRY(mark(5));
     if (split)
-        HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, h->d_tile_start, h->d_pval[pi],
+        HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start,
+                                 h->d_pval[pi],
                                  h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
                                  h->blend_batch));
     HIP_TRY(mark(6));
@@ -754,6 +756,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     h->last.width = wh[0];
     h->last.height = wh[1];
     h->last.tiles = total_tiles;
+    h->last_sbs = (uint32_t)total_sb;
     h->last.visible = vis_sum;
     uint64_t pair_sum = 0;
     for (uint32_t v = 0; v < nviews; ++v) pair_sum += vpairs[v];
@@ -1544,6 +1547,7 @@ int sp_diff_forward(sp_handle *h, const sp_params *params, const sp_view *views,
     if (rc) return rc;
     h->fwd_views = n_views;
     h->fwd_tiles = (uint32_t)h->last.tiles;
+    h->fwd_sbs = h->last_sbs;
     h->fwd_pairs = h->last.pairs;
     return SP_OK;
 }
@@ -1578,7 +1582,7 @@ int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *
         h->stamp_id = 1;
     }
     HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, (uint64_t)nv * h->n * sp::NG2D, 0.0f));  // the records in use
-    HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->d_tile_start, h->d_pval[h->pair_buf],
+    HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->fwd_sbs, h->d_tile_start, h->d_pval[h->pair_buf],
                                  h->d_pbase, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
                                  h->d_stamp, h->stamp_id));
     HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_stamp, h->stamp_id, h->d_powner, np, h->d_g2d,

@@ -38,23 +38,19 @@
 namespace sp {
 namespace {
 
-constexpr int XCDS = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs
-
 template <int BATCH, int MODE>
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
                                                          uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
-                                                         const uint4 *__restrict__ rec, uint32_t per_xcd,
+                                                         const uint4 *__restrict__ rec, uint32_t total_sbs,
                                                          uint32_t *__restrict__ counters, float *__restrict__ pix_T,
                                                          uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
                                                          float4 *__restrict__ pstate) {
     __shared__ uint4 s_rec[BATCH][3];
     __shared__ uint32_t s_qm[BATCH];  // reach bits (0-3) and full-cover bits (4-7) of each staged record
-    // Consecutive tiles (rows of an image) on the same XCD share its L2.
-    const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
-    if (gt >= total_tiles) return;
+    uint32_t gt, lo;
+    if (!blend_tile_of(views, nviews, total_sbs, blockIdx.x, &gt, &lo)) return;
     if (MODE == BM_SECOND && done[gt]) return;  // finished by the first list
-    const uint32_t lo = view_of_tile(views, nviews, gt);
     const ViewCtx &v = views[lo].ctx;
     uint8_t *__restrict__ rgb = views[lo].out;
     const uint32_t t = gt - views[lo].tile_base;
@@ -198,23 +194,22 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
 }
 
 template <int MODE>
-void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles, uint32_t total_sbs,
                  const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                  uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate, int batch) {
-    const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
-    const dim3 grid(per_xcd * XCDS), block(BLEND_THREADS);
+    const dim3 grid(blend_grid(total_sbs)), block(BLEND_THREADS);
     switch (batch) {
     case 64:
         hipLaunchKernelGGL((k_blend<64, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
         break;
     case 128:
         hipLaunchKernelGGL((k_blend<128, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
         break;
     default:
         hipLaunchKernelGGL((k_blend<256, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, per_xcd, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
         break;
     }
 }
@@ -222,26 +217,26 @@ void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32
 }  // namespace
 
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
+                        uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch) {
     if (total_tiles == 0) return hipSuccess;
     float4 *ps = reinterpret_cast<float4 *>(pstate);
     switch (mode) {
     case BM_DIFF:
-        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T, pix_last,
+        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
                              done, ps, batch);
         break;
     case BM_FIRST:
-        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T,
+        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
                               pix_last, done, ps, batch);
         break;
     case BM_SECOND:
-        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T,
+        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
                                pix_last, done, ps, batch);
         break;
     default:
-        launch_mode<BM_RGB>(s, views, nviews, total_tiles, tile_start, sval, rec, counters, pix_T, pix_last,
+        launch_mode<BM_RGB>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
                             done, ps, batch);
         break;
     }

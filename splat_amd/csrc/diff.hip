@@ -44,7 +44,6 @@
 namespace sp {
 namespace {
 
-constexpr int XCDS = 8;
 constexpr int BWD_BATCH = 64;  // records per LDS step of the adjoint blend
 constexpr int SR_THREADS = 256;
 constexpr int PBWD_THREADS = 128;
@@ -103,15 +102,14 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ pbase,
     const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
     float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ stamp, uint32_t stamp_id,
-    uint32_t per_xcd) {
+    uint32_t total_sbs) {
     __shared__ uint4 s_rec[BWD_BATCH][3];
     __shared__ uint32_t s_e[BWD_BATCH];
     __shared__ uint32_t s_qm[BWD_BATCH];  // quadrant_mask of each staged record
     __shared__ float s_part[4][NG2D][BWD_BATCH];
     __shared__ uint32_t s_wmax[4];
-    const uint32_t gt = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
-    if (gt >= total_tiles) return;
-    const uint32_t vi = view_of_tile(views, nviews, gt);
+    uint32_t gt, vi;
+    if (!blend_tile_of(views, nviews, total_sbs, blockIdx.x, &gt, &vi)) return;
     const ViewCtx &v = views[vi].ctx;
     const uint32_t t = gt - views[vi].tile_base;
     const uint32_t tx = t % v.tiles_x, ty = t / v.tiles_x;
@@ -617,13 +615,14 @@ void launch_pbwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uin
 }  // namespace
 
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
+                            uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval,
+                            const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id) {
     if (total_tiles == 0) return hipSuccess;
-    const uint32_t per_xcd = (total_tiles + XCDS - 1) / XCDS;
-    hipLaunchKernelGGL(k_blend_bwd, dim3(per_xcd * XCDS), dim3(BLEND_THREADS), 0, s, views, nviews, total_tiles,
-                       tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id, per_xcd);
+    hipLaunchKernelGGL(k_blend_bwd, dim3(blend_grid(total_sbs)), dim3(BLEND_THREADS), 0, s, views, nviews,
+                       total_tiles, tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id,
+                       total_sbs);
     return hipGetLastError();
 }
 

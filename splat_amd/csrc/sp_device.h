@@ -116,6 +116,40 @@ __device__ __forceinline__ uint32_t view_of_tile(const ViewParams *views, uint32
     return lo;
 }
 
+// The tile of blend workgroup b. The dispatcher deals workgroups to the 8
+// XCDs round robin (b % 8), whatever their cost, so the tiles of one XCD must
+// carry an even share of the frame's work: XCD x takes the 4x4-tile
+// superblocks x, x + 8, x + 16, ... of the batch (numbered as the views'
+// sb_base numbers them), one tile per workgroup, the 16 tiles of a
+// superblock on consecutive workgroups so that they share the XCD's L2.
+// Workgroups of tiles past a frame's edge, or past the last superblock,
+// return false. The grid is blend_grid(total_sbs) workgroups.
+constexpr uint32_t BLEND_XCDS = 8;
+__host__ __device__ constexpr uint32_t blend_grid(uint32_t total_sbs) {
+    return BLEND_XCDS * SB * SB * ((total_sbs + BLEND_XCDS - 1) / BLEND_XCDS);
+}
+__host__ __device__ inline bool blend_tile_of(const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
+                                              uint32_t b, uint32_t *gt, uint32_t *vi) {
+    const uint32_t j = b / BLEND_XCDS;
+    const uint32_t sbi = b % BLEND_XCDS + BLEND_XCDS * (j / (SB * SB)), q = j % (SB * SB);
+    if (sbi >= total_sbs) return false;
+    uint32_t lo = 0, hi = nviews;  // the last view whose sb_base <= sbi
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].sb_base <= sbi)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const ViewCtx &v = views[lo].ctx;
+    const uint32_t ls = sbi - views[lo].sb_base, sbx = (v.tiles_x + SB - 1) / SB;
+    const uint32_t tx = (ls % sbx) * SB + q % SB, ty = (ls / sbx) * SB + q / SB;
+    if (tx >= v.tiles_x || ty >= v.tiles_y) return false;
+    *gt = views[lo].tile_base + ty * v.tiles_x + tx;
+    *vi = lo;
+    return true;
+}
+
 // Sum of v over the 64 lanes, left in lane 63, with six DPP adds (no LDS
 // traffic): pairs and quads (quad_perm), eights (row_half_mirror), rows of 16
 // (row_mirror), then rows 0+1 and 2+3 (row_bcast:15 into rows 1, 3) and all

@@ -131,7 +131,7 @@ hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask,
 // pstate (total_tiles * 256 * 4 floats); BM_SECOND resumes from it.
 // batch = records staged through LDS per step (64, 128 or 256).
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                        const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
+                        uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
                         int batch);
 
@@ -142,7 +142,7 @@ hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32
 // position, each written pair stamped with stamp_id; pairs after it are left
 // untouched.
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
-                            const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
+                            uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id);
 // Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.
