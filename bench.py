@@ -33,7 +33,7 @@ Also reported beside the headline value:
 Only the cpu_baseline leg runs the oracle (as the CPU baseline and to check
 the view it times byte for byte); every other figure is the HIP path's.
 
-python bench.py [--gpus 1] [--steps 40] [--warmup 5]
+python bench.py [--gpus 1] [--steps 200] [--warmup 80]
 """
 from __future__ import annotations
 
@@ -71,8 +71,8 @@ def log(msg: str) -> None:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=80)
     ap.add_argument("--splats", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -133,7 +133,7 @@ def main() -> None:
         tuned = r.tuning()
     else:
         log("autotune on one orbit view")
-        tuned = r.autotune(views[:1], [(W, H)], bg, reps=5)
+        tuned = r.autotune(views[:1], [(W, H)], bg, reps=9)
     log(f"tuning {tuned}")
     log("warmup")
     for i in range(max(args.warmup, 1)):

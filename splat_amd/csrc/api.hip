@@ -180,7 +180,7 @@ struct sp_handle {
     sp_frame_info last{};
     int pair_buf = 0;
     int depth_buf = 0;  // ping-pong buffer holding the last depth sort's output
-    int blend_batch = 256;  // records staged through LDS per blend step (64, 128 or 256)
+    int blend_batch = 64;  // records staged through LDS per blend step (64, 128 or 256)
     // Front-to-back split of the plain path: the first phase emits each view's
     // nearest max(split_kmin, vis >> split_shift) splats (split_shift 0: no split).
     uint32_t split_shift = 3, split_kmin = 16384;

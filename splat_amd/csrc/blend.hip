@@ -38,16 +38,95 @@
 namespace sp {
 namespace {
 
+// A record as staged for one tile. The power at pixel (column c, row r) of the
+// tile is (col[c].y + row[r].x) + row[r].y * col[c].x, the reference's
+// (qx + qy) + (-b dy) dx with every factor rounded as it rounds them, so a
+// visit needs three operations for it. A column or row outside the record's
+// pixel box stores a NaN in place of qx or qy: the power is then NaN, which
+// fails `pw <= 0`, so the pixel is masked as the reference masks it by the box
+// (whatever the opacity), and the threshold compare skips it too.
+// 304 bytes, an odd number of 16-byte units, so the 16-byte writes of eight
+// consecutive staging lanes fall in distinct LDS bank groups.
+struct StagedRec {
+    float2 col[TILE];  // (dx, qx = (dx * dx) * (-a/2)) per tile column
+    float2 row[TILE];  // (qy = ((-c/2) * dy) * dy, -b * dy) per tile row
+    float4 orgb;       // opacity, colour
+    float thr, pad0, pad1, pad2;
+    float4 pad3;
+};
+static_assert(sizeof(StagedRec) == 304, "staged record size");
+
+// Stages part `part` of P of one record for the tile at (ox, oy): its
+// columns and rows [part * 16/P, (part + 1) * 16/P) and the reach bits of its
+// quadrants [part * 4/P, (part + 1) * 4/P) (one byte per quadrant in qb), so
+// the P lanes of a record share the work and every wave of the tile stages.
+// The offsets are the reference's (src/render/cpu.rs:658-672):
+// cx0 = (ox + 0.5) - x, dx = lane + (4*group + cx0) per column,
+// dy = ((oy + r) + 0.5) - y per row. Frame coordinates are below 2^16, so
+// (oy + r0) + (j + 0.5) is exact and equals (oy + r) + 0.5 for r = r0 + j.
+template <int P>
+__device__ __forceinline__ void stage_part(StagedRec &d, uint8_t *qb, const uint4 &r0, const uint4 &r1,
+                                           const uint4 &r2, uint32_t ox, uint32_t oy, uint32_t part) {
+    constexpr int NC = (int)TILE / P, NQ = 4 / P;
+    const int c0 = (int)part * NC;
+    const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
+    const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w), nc = __uint_as_float(r1.x);
+    const float thr = __uint_as_float(r2.w);
+    const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
+    const int lx0 = (int)bx0 - (int)ox, lx1 = (int)bx1 - (int)ox;
+    const int ly0 = (int)by0 - (int)oy, ly1 = (int)by1 - (int)oy;
+    const float cx0 = ((float)ox + 0.5f) - sx;
+    const float qnan = __builtin_nanf("");
+#pragma unroll
+    for (int gi = 0; gi < NC; gi += 4) {
+        const int c = c0 + gi;             // a multiple of 4: the first column of a 4-lane group
+        const float g = (float)c + cx0;    // 4 * group + cx0
+#pragma unroll
+        for (int j = 0; j < 4; j += 2) {
+            const float dxa = (float)j + g, dxb = (float)(j + 1) + g;
+            const float qa = (dxa * dxa) * na, qb2 = (dxb * dxb) * na;
+            *reinterpret_cast<float4 *>(&d.col[c + j]) =
+                make_float4(dxa, (c + j >= lx0 && c + j < lx1) ? qa : qnan, dxb,
+                            (c + j + 1 >= lx0 && c + j + 1 < lx1) ? qb2 : qnan);
+        }
+    }
+    const float fy = (float)(oy + (uint32_t)c0);
+#pragma unroll
+    for (int j = 0; j < NC; j += 2) {
+        const int r = c0 + j;
+        const float dya = (fy + ((float)j + 0.5f)) - sy, dyb = (fy + ((float)j + 1.5f)) - sy;
+        const float qa = (nc * dya) * dya, qb2 = (nc * dyb) * dyb;
+        *reinterpret_cast<float4 *>(&d.row[r]) = make_float4((r >= ly0 && r < ly1) ? qa : qnan, nb2 * dya,
+                                                             (r + 1 >= ly0 && r + 1 < ly1) ? qb2 : qnan, nb2 * dyb);
+    }
+    if (part == 0) {
+        d.orgb = make_float4(__uint_as_float(r1.y), __uint_as_float(r1.z), __uint_as_float(r1.w),
+                             __uint_as_float(r2.x));
+        d.thr = thr;
+    }
+    const float ia = 1.0f / (2.0f * na), ic = 1.0f / (2.0f * nc);
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k)
+        if (quadrant_reaches(sx, sy, na, nb2, nc, ia, ic, thr, bx0, by0, bx1, by1, ox, oy, part * NQ + k))
+            bits |= 1u << (8 * k);
+    if constexpr (NQ == 4) *reinterpret_cast<uint32_t *>(qb) = bits;
+    else if constexpr (NQ == 2) *reinterpret_cast<uint16_t *>(qb + 2 * part) = (uint16_t)bits;
+    else qb[part] = (uint8_t)bits;
+}
+
 template <int BATCH, int MODE>
-__global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
+__global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(BATCH == 64 ? 8 : 1))) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
                                                          uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
                                                          const uint4 *__restrict__ rec, uint32_t total_sbs,
                                                          uint32_t *__restrict__ counters, float *__restrict__ pix_T,
                                                          uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
                                                          float4 *__restrict__ pstate) {
-    __shared__ uint4 s_rec[BATCH][3];
-    __shared__ uint32_t s_qm[BATCH];  // reach bits (0-3) and full-cover bits (4-7) of each staged record
+    constexpr int P = BLEND_THREADS / BATCH;  // lanes staging each record
+    static_assert(P * BATCH == BLEND_THREADS && (P == 1 || P == 2 || P == 4), "staging split");
+    __shared__ StagedRec s_st[BATCH];
+    __shared__ uint8_t s_qb[BATCH][4];  // reach bit of each staged record per quadrant
     uint32_t gt, lo;
     if (!blend_tile_of(views, nviews, total_sbs, blockIdx.x, &gt, &lo)) return;
     if (MODE == BM_SECOND && done[gt]) return;  // finished by the first list
@@ -62,10 +141,6 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     const uint32_t px = qx0 + (lane_i & 7u), py = qy0 + (lane_i >> 3);
     const uint32_t lx = px - ox, ly = py - oy;
     const bool inside = px < v.w && py < v.h;
-    const float fx = (float)ox;
-    const float gx = (float)(lx & ~3u);  // 4 * group of the reference's 4-lane groups
-    const float lane = (float)(lx & 3u);
-    const float pyc = ((float)oy + (float)ly) + 0.5f;
     const uint64_t pix = (uint64_t)gt * BLEND_THREADS + threadIdx.x;
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     if (MODE == BM_SECOND) {
@@ -86,19 +161,12 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
     const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
     uint32_t b = start;
     for (; b < end; b += BATCH) {
-        const uint32_t idx = b + threadIdx.x;
-        if (threadIdx.x < BATCH && idx < end) {
-            uint32_t id = sval[idx];
+        const uint32_t rr = threadIdx.x / P, idx = b + rr;
+        if (idx < end) {
+            const uint32_t id = sval[idx];
             const uint4 *r = rec + 3 * (uint64_t)id;
-            uint4 r0 = r[0];
-            const uint4 r1 = r[1], r2 = r[2];
-            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy) | (cover_mask(r2, ox, oy) << 4);
-            // The staged x is the tile's ((ox + 0.5) - x), the reference's
-            // per-record term of every pixel offset, computed once here.
-            r0.x = __float_as_uint((fx + 0.5f) - __uint_as_float(r0.x));
-            s_rec[threadIdx.x][0] = r0;
-            s_rec[threadIdx.x][1] = r1;
-            s_rec[threadIdx.x][2] = r2;
+            const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
+            stage_part<P>(s_st[rr], s_qb[rr], r0, r1, r2, ox, oy, threadIdx.x % P);
         }
         __syncthreads();
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
@@ -106,54 +174,38 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend(const ViewParams *__res
         // time: one LDS read per lane and a ballot, then the set bits in order.
         bool open = live_mask() != 0ull;
         for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
-            const uint32_t qm = g0 + lane_i < nb ? s_qm[g0 + lane_i] : 0u;
-            unsigned long long reach = lane_mask((qm >> w) & 1u);
-            // Records whose box holds the whole quadrant: no box test.
-            const unsigned long long cover = lane_mask((qm >> (4 + w)) & 1u);
+            const uint32_t qm = g0 + lane_i < nb ? s_qb[g0 + lane_i][w] : 0u;
+            unsigned long long reach = lane_mask(qm != 0u);
             while (reach) {
+                // Live lanes as a wave mask (the per-lane conditions below are
+                // all wave masks, combined on the scalar unit).
                 const uint64_t lm = live_mask();
                 if (lm == 0ull) {  // every pixel of the quadrant saturated
                     open = false;
                     break;
                 }
-                const uint32_t kb = (uint32_t)__builtin_ctzll(reach);
-                const uint32_t k = g0 + kb;
+                const uint32_t k = g0 + (uint32_t)__builtin_ctzll(reach);
                 reach &= reach - 1ull;
-                const uint4 r0 = s_rec[k][0];
-                const uint4 r1 = s_rec[k][1];
-                const uint4 r2 = s_rec[k][2];
-                // All 48 bytes are read at once from one address register (the
-                // compiler would otherwise sink each field's read to its first
-                // use and rebuild the address for each, one VALU move apiece).
-                asm volatile("" ::"v"(r0.x), "v"(r0.y), "v"(r0.z), "v"(r0.w), "v"(r1.x), "v"(r1.y), "v"(r1.z),
-                             "v"(r1.w), "v"(r2.x), "v"(r2.y), "v"(r2.z), "v"(r2.w));
-                // Live in-box lanes as a wave mask (the per-lane conditions below
-                // are all wave masks, combined on the scalar unit).
-                uint64_t act_mask = lm;
-                if (!((cover >> kb) & 1ull)) {  // wave-uniform
-                    const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-                    act_mask &= lane_mask(px >= bx0) & lane_mask(px < bx1) & lane_mask(py >= by0) & lane_mask(py < by1);
-                }
-                const float cx0 = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
-                const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w);
-                const float nc = __uint_as_float(r1.x);
-                const float dx = lane + (gx + cx0);
-                const float dy = pyc - sy;
-                const float qx = (dx * dx) * na;
-                const float qy = (nc * dy) * dy;
-                const float bdy = nb2 * dy;
-                const float pw = (qx + qy) + bdy * dx;
+                const StagedRec &sr = s_st[k];
+                const float2 cv = sr.col[lx];
+                const float2 rv = sr.row[ly];
+                const float4 c4 = sr.orgb;
+                const float thr = sr.thr;
+                // The record's five wave-uniform words are read at once from one
+                // address register (the compiler would otherwise sink each read
+                // to its first use and rebuild the address for each).
+                asm volatile("" ::"v"(c4.x), "v"(c4.y), "v"(c4.z), "v"(c4.w), "v"(thr));
+                const float pw = (cv.y + rv.x) + rv.y * cv.x;
                 // Lanes below the splat's threshold have alpha < ALPHA_MIN (the
                 // reference masks them); a wave with none left skips the rest.
-                if ((act_mask & lane_mask(pw >= __uint_as_float(r2.w))) == 0ull) continue;
-                const float op = __uint_as_float(r1.y);
-                const float alpha = fminf(exp_neg_times(pw, op), ALPHA_MAX);
-                const uint64_t upd = act_mask & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                if ((lm & lane_mask(pw >= thr)) == 0ull) continue;
+                const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
+                const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
                 if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
                     const float wgt = alpha * T;
-                    cr = cr + __uint_as_float(r1.z) * wgt;
-                    cg = cg + __uint_as_float(r1.w) * wgt;
-                    cb = cb + __uint_as_float(r2.x) * wgt;
+                    cr = cr + c4.y * wgt;
+                    cg = cg + c4.z * wgt;
+                    cb = cb + c4.w * wgt;
                     T = T * (1.0f - alpha);
                     if (MODE == BM_DIFF) last = (b - start) + k + 1;
                 }

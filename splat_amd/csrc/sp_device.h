@@ -178,20 +178,6 @@ __device__ __forceinline__ uint32_t quadrant_mask(const uint4 &r0, const uint4 &
     return quadrant_mask_words(w0, w1, w2, ox, oy);
 }
 
-// Bit q set when the record's pixel box holds the whole 8x8 quadrant
-// (q & 1, q >> 1) of the tile at ox, oy (then every pixel of the quadrant is
-// in the box, and the blend needs no per-pixel box test).
-__device__ __forceinline__ uint32_t cover_mask(const uint4 &r2, uint32_t ox, uint32_t oy) {
-    const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-    uint32_t m = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint32_t qx0 = ox + 8u * (q & 1), qy0 = oy + 8u * (q >> 1);
-        if (bx0 <= qx0 && bx1 >= qx0 + 8u && by0 <= qy0 && by1 >= qy0 + 8u) m |= 1u << q;
-    }
-    return m;
-}
-
 // Rust `as u32` on f32 (saturating, NaN -> 0).
 __device__ __forceinline__ uint32_t sat_u32(float v) {
     if (!(v > 0.0f)) return 0u;

@@ -173,7 +173,7 @@ class HipRenderer:
         N.check(self._lib.sp_get_tuning(self._h, C.byref(t)))
         return {"blend_batch": t.blend_batch, "split_shift": t.split_shift, "split_kmin": t.split_kmin}
 
-    def set_tuning(self, blend_batch: int = 256, split_shift: int = 3, split_kmin: int = 16384) -> None:
+    def set_tuning(self, blend_batch: int = 64, split_shift: int = 3, split_kmin: int = 16384) -> None:
         t = N.SpTuning(blend_batch, split_shift, split_kmin, 0)
         N.check(self._lib.sp_set_tuning(self._h, C.byref(t)))
 
