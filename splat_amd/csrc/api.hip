@@ -336,6 +336,29 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
     return ensure_table(h);
 }
 
+// Pair capacity of a new handle. The split plain path materialises only the
+// first phase's pairs at once.
+uint64_t initial_pairs(const sp_handle *h) {
+    return h->opt.initial_pairs ? h->opt.initial_pairs
+                                : std::max<uint64_t>((h->diff ? 16ull : 4ull) * h->n, 1ull << 20);
+}
+
+// Returns the tile-pair buffers to a new handle's capacity, so later frames
+// grow them as a new handle's would. The pair sorts and range searches size
+// their grids and radix tiles from the capacity, so a capacity left far above
+// the need (after a frame that sorted every pair of an unsplit pipeline) slows
+// every later frame of a smaller need.
+int reset_pairs(sp_handle *h) {
+    if (h->fixed) return SP_OK;
+    for (int b = 0; b < 2; ++b) {
+        dfree(h, h->d_pkey[b], h->pair_cap * 4);
+        dfree(h, h->d_pval[b], h->pair_cap * 4);
+    }
+    dfree(h, h->d_powner, h->pair_cap * 4);
+    h->pair_cap = 0;
+    return ensure_pairs(h, initial_pairs(h));
+}
+
 // Per-view arrays for `views` views.
 int ensure_views(sp_handle *h, uint32_t views) {
     if (views <= h->view_cap) return SP_OK;
@@ -720,7 +743,11 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             return fail(SP_E_CAPACITY, "more than 2^32 (gaussian, tile) pairs in one phase of the frame");
         if (h->h_counters[sp::CNT_OVERFLOW] && attempt < 3) {
             uint64_t p = h->h_counters[sp::CNT_PAIRS];
-            if (mode == BL_RGB && h->split_shift > 0) p = std::max<uint64_t>(p, h->h_counters[sp::CNT_PAIRS2]);
+            // The second phase's count means something only when the first
+            // phase fitted: after a first phase that overflowed, its blend saw
+            // partial lists and left nearly every tile open.
+            if (mode == BL_RGB && h->split_shift > 0 && p <= h->pair_cap)
+                p = std::max<uint64_t>(p, h->h_counters[sp::CNT_PAIRS2]);
             rc = ensure_pairs(h, p + p / 4 + 1024);
             if (rc) return rc;
             continue;
@@ -861,10 +888,7 @@ int open_handle(sp_handle *h) {
 // Counters, one view's arrays and the initial pair capacity.
 int finish_handle(sp_handle *h) {
     int rc = ensure_views(h, 1);
-    // The split plain path materialises only the first phase's pairs at once.
-    const uint64_t cap0 = h->opt.initial_pairs ? h->opt.initial_pairs
-                                               : std::max<uint64_t>((h->diff ? 16ull : 4ull) * h->n, 1ull << 20);
-    if (!rc) rc = ensure_pairs(h, cap0);
+    if (!rc) rc = ensure_pairs(h, initial_pairs(h));
     return rc;
 }
 
@@ -1439,8 +1463,13 @@ int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t
     reps = std::max(1, std::min(reps, 50));
     sp_tuning cur;
     sp_get_tuning(h, &cur);
+    // Each candidate starts from a new handle's pair capacity, which its
+    // untimed first frame grows to its own need: a candidate that sorts every
+    // pair unsplit would otherwise leave capacity-sized grids to the ones after.
     auto time_of = [&](const sp_tuning &t, double *ms) -> int {
         int r = sp_set_tuning(h, &t);
+        if (r) return r;
+        r = reset_pairs(h);
         if (r) return r;
         r = run_views(h, nb, bv.data(), bwh.data(), background, outs.data(), BL_RGB, true, nullptr);
         if (r) return r;
@@ -1488,6 +1517,8 @@ int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t
             if ((rc = consider(t))) return rc;
         }
     rc = sp_set_tuning(h, &cur);
+    if (rc) return rc;
+    rc = reset_pairs(h);  // later frames grow them to the chosen setting's need
     if (rc) return rc;
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
