@@ -1,18 +1,20 @@
 // (e) Per-tile front-to-back alpha blending.
 //
 // One 256-thread workgroup per 16x16 tile of any view of the batch (the views'
-// tiles are numbered consecutively), one pixel per thread; wave k of the
-// group owns one 8x8 quadrant. The tile's depth-ordered splat list is staged
-// through LDS BATCH records at a time (each thread gathers one 48-byte
-// record and bounds its falloff over each quadrant), then every wave walks
-// the batch: a record whose pixel box misses the wave's quadrant, or whose
-// falloff cannot reach the splat's alpha threshold at any in-box pixel of the
-// quadrant, is never visited (the wave ballots the reach bits of 64 staged
-// records at a time and walks only the set bits, in list order); one whose
-// falloff stays below the threshold at every live in-box pixel is skipped
-// after the falloff test (those pixels are masked by the reference too).
-// A pixel stops at T < T_MIN; a wave stops when its quadrant has no live
-// pixel, the tile when no pixel is live.
+// tiles are numbered consecutively, and blend_tile_of deals whole 4x4-tile
+// superblocks to the XCDs in turn), one pixel per thread; wave k of the group
+// owns one 8x8 quadrant. The tile's depth-ordered splat list is staged through
+// LDS BATCH records at a time: 256 / BATCH lanes gather each 48-byte record,
+// write its per-column and per-row power terms for this tile and bound its
+// falloff over each quadrant. Then every wave walks the batch: a record whose
+// pixel box misses the wave's quadrant, or whose falloff cannot reach the
+// splat's alpha threshold at any in-box pixel of the quadrant, is never
+// visited (the wave ballots the reach bits of 64 staged records at a time and
+// walks only the set bits, in list order); one whose falloff stays below the
+// threshold at every live in-box pixel is skipped after the falloff test
+// (those pixels are masked by the reference too). A pixel stops at
+// T < T_MIN; a wave stops when its quadrant has no live pixel, the tile when
+// no pixel is live.
 //
 // Per-pixel arithmetic is raster_tile's (src/render/cpu.rs:609-719) in the
 // same operation order, with pixel offsets built from the 16-aligned tile
