@@ -144,20 +144,20 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
     const uint32_t lx = px - ox, ly = py - oy;
     const bool inside = px < v.w && py < v.h;
     const uint64_t pix = (uint64_t)gt * BLEND_THREADS + threadIdx.x;
-    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    // Lanes outside the frame carry T = 0, so a pixel is live exactly when
+    // T >= T_MIN (T only falls and is never NaN, so this is the reference's
+    // stop at T < T_MIN for the frame's pixels): one compare, no loop-carried
+    // mask. An outside lane's T is never written out as its own.
+    float T = inside ? 1.0f : 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     if (MODE == BM_SECOND) {
         const float4 st = pstate[pix];
-        T = st.x;
+        T = inside ? st.x : 0.0f;
         cr = st.y;
         cg = st.z;
         cb = st.w;
     }
-    // A pixel is live while inside the frame with T >= T_MIN (T only falls and
-    // is never NaN, so this is the reference's stop at T < T_MIN); deriving it
-    // from T keeps it a plain compare rather than a loop-carried mask.
-    const auto live = [&]() { return inside && T >= T_MIN; };
-    const uint64_t inside_mask = lane_mask(inside);
-    const auto live_mask = [&]() { return lane_mask(T >= T_MIN) & inside_mask; };
+    const auto live = [&]() { return T >= T_MIN; };
+    const auto live_mask = [&]() { return lane_mask(T >= T_MIN); };
     uint32_t last = 0;
     bool finished = false;
     const uint32_t start = tile_start[gt], end = tile_start[gt + 1];
@@ -174,20 +174,19 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
         // Records of the batch that can reach this wave's quadrant, 64 at a
         // time: one LDS read per lane and a ballot, then the set bits in order.
-        bool open = live_mask() != 0ull;
-        for (uint32_t g0 = 0; g0 < nb && open; g0 += 64) {
+        // Live lanes as a wave mask (the per-lane conditions below are all
+        // wave masks, combined on the scalar unit); it changes only when a
+        // record updates T, and the wave stops once its quadrant saturated.
+        uint64_t lm = live_mask();
+        for (uint32_t g0 = 0; g0 < nb && lm != 0ull; g0 += 64) {
             const uint32_t qm = g0 + lane_i < nb ? s_qb[g0 + lane_i][w] : 0u;
             unsigned long long reach = lane_mask(qm != 0u);
             while (reach) {
-                // Live lanes as a wave mask (the per-lane conditions below are
-                // all wave masks, combined on the scalar unit).
-                const uint64_t lm = live_mask();
-                if (lm == 0ull) {  // every pixel of the quadrant saturated
-                    open = false;
-                    break;
-                }
-                const uint32_t k = g0 + (uint32_t)__builtin_ctzll(reach);
-                reach &= reach - 1ull;
+                const uint32_t kb = (uint32_t)__builtin_ctzll(reach);
+                const uint32_t k = g0 + kb;
+                // Clears the visited bit: one scalar instruction instead of the
+                // three of a 64-bit reach &= reach - 1.
+                asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kb));
                 const StagedRec &sr = s_st[k];
                 const float2 cv = sr.col[lx];
                 const float2 rv = sr.row[ly];
@@ -200,16 +199,19 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
                 const float pw = (cv.y + rv.x) + rv.y * cv.x;
                 // Lanes below the splat's threshold have alpha < ALPHA_MIN (the
                 // reference masks them); a wave with none left skips the rest.
-                if ((lm & lane_mask(pw >= thr)) == 0ull) continue;
-                const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
-                const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
-                if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
-                    const float wgt = alpha * T;
-                    cr = cr + c4.y * wgt;
-                    cg = cg + c4.z * wgt;
-                    cb = cb + c4.w * wgt;
-                    T = T * (1.0f - alpha);
-                    if (MODE == BM_DIFF) last = (b - start) + k + 1;
+                if ((lm & lane_mask(pw >= thr)) != 0ull) {
+                    const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
+                    const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                    if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
+                        const float wgt = alpha * T;
+                        cr = cr + c4.y * wgt;
+                        cg = cg + c4.z * wgt;
+                        cb = cb + c4.w * wgt;
+                        T = T * (1.0f - alpha);
+                        if (MODE == BM_DIFF) last = (b - start) + k + 1;
+                    }
+                    lm = live_mask();
+                    if (lm == 0ull) reach = 0ull;  // every pixel of the quadrant saturated
                 }
             }
         }
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
         if (threadIdx.x == 0) done[gt] = 1u;
     }
     if (MODE == BM_DIFF) {
-        pix_T[pix] = T;
+        pix_T[pix] = inside ? T : 1.0f;
         pix_last[pix] = last;
         if (inside) {
             float *o = views[lo].fimg + 3 * ((uint64_t)py * v.w + px);
