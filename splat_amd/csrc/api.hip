@@ -185,7 +185,7 @@ struct sp_handle {
     // nearest max(split_kmin, vis >> split_shift) splats (split_shift 0: no split).
     uint32_t split_shift = 3, split_kmin = 16384;
     uint32_t *d_done = nullptr;  // split_cap tiles: finished by the first phase
-    uint32_t *d_ip = nullptr;    // split_cap tiles: row prefix counts of unfinished tiles
+    uint32_t *d_openm = nullptr;  // split_cap words: per-row bits of the tiles the first blend left open
     float *d_pstate = nullptr;   // split_cap * 256 pixels: (T, r, g, b) between the phases
     uint64_t split_cap = 0;
     // Differentiable path (sp_diff_create): raw parameters on the caller's
@@ -477,11 +477,11 @@ int ensure_split(sp_handle *h, uint64_t tiles) {
     if (tiles <= h->split_cap) return SP_OK;
     if (h->fixed) return fixed_full("tile states");
     dfree(h, h->d_done, h->split_cap * 4);
-    dfree(h, h->d_ip, h->split_cap * 4);
+    dfree(h, h->d_openm, h->split_cap * 4);
     dfree(h, h->d_pstate, h->split_cap * sp::BLEND_THREADS * 16);
     h->split_cap = 0;
     int rc = dalloc(h, (void **)&h->d_done, tiles * 4);
-    if (!rc) rc = dalloc(h, (void **)&h->d_ip, tiles * 4);
+    if (!rc) rc = dalloc(h, (void **)&h->d_openm, tiles * 4);
     if (!rc) rc = dalloc(h, (void **)&h->d_pstate, tiles * sp::BLEND_THREADS * 16);
     if (rc) return rc;
     h->split_cap = tiles;
@@ -593,7 +593,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
     uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
-                            es, nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs,
+                            es, split ? h->d_openm : nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all,
+                            vtot, view_pairs,
                             h->d_bsb, vsb, h->d_pkey[0], h->d_pval[0], powner, powner ? h->d_pbase : nullptr,
                             h->pair_cap, h->d_counters));
     HIP_TRY(mark(1));
@@ -618,7 +619,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
         HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start, h->d_pval[pi],
                                  h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast, h->d_done, h->d_pstate,
-                                 h->blend_batch));
+                                 split ? h->d_openm : nullptr, split ? vopen : nullptr, h->blend_batch));
     }
     HIP_TRY(mark(3));
     if (after_first) {
@@ -626,9 +627,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         if (rc) return rc;
     }
     if (split) {
-        HIP_TRY(sp::launch_row_prefix(s, h->d_views, nviews, h->d_done, h->d_ip, vopen));
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
-                                h->d_views, es, h->d_ip, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
+                                h->d_views, es, h->d_openm, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
                                 nullptr, h->d_pkey[0], h->d_pval[0], nullptr, nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));
@@ -642,8 +642,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     if (split)
         HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start,
                                  h->d_pval[pi],
-                                 h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate,
-                                 h->blend_batch));
+                                 h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate, nullptr,
+                                 nullptr, h->blend_batch));
     HIP_TRY(mark(6));
     // Counters, visible counts and pair counts (view_cap apart) in one copy.
     HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, (sp::CNT_SLOTS + h->view_cap + nviews) * 4,
@@ -976,7 +976,7 @@ uint64_t carve_frame(sp_handle *t, uint8_t *base, uint32_t width, uint32_t heigh
     t->d_trow = c.take<uint32_t>(rows * 4);
     t->split_cap = t->split_shift > 0 ? tiles : 0;
     t->d_done = t->split_shift > 0 ? c.take<uint32_t>(tiles * 4) : nullptr;
-    t->d_ip = t->split_shift > 0 ? c.take<uint32_t>(tiles * 4) : nullptr;
+    t->d_openm = t->split_shift > 0 ? c.take<uint32_t>(tiles * 4) : nullptr;
     t->d_pstate = t->split_shift > 0 ? c.take<float>(tiles * sp::BLEND_THREADS * 16) : nullptr;
     t->rgb_cap = 0;
     t->d_rgb = nullptr;
@@ -1037,7 +1037,7 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_g2d);
     (void)hipFree(h->d_part);
     (void)hipFree(h->d_done);
-    (void)hipFree(h->d_ip);
+    (void)hipFree(h->d_openm);
     (void)hipFree(h->d_pstate);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->h_views) (void)hipHostFree(h->h_views);

@@ -124,7 +124,8 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
                                                          const uint4 *__restrict__ rec, uint32_t total_sbs,
                                                          uint32_t *__restrict__ counters, float *__restrict__ pix_T,
                                                          uint32_t *__restrict__ pix_last, uint32_t *__restrict__ done,
-                                                         float4 *__restrict__ pstate) {
+                                                         float4 *__restrict__ pstate, uint32_t *__restrict__ openm,
+                                                         uint32_t *__restrict__ vopen) {
     constexpr int P = BLEND_THREADS / BATCH;  // lanes staging each record
     static_assert(P * BATCH == BLEND_THREADS && (P == 1 || P == 2 || P == 4), "staging split");
     __shared__ StagedRec s_st[BATCH];
@@ -227,7 +228,12 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
         if (!finished) finished = __syncthreads_count(live()) == 0;
         if (!finished) {
             pstate[pix] = make_float4(T, cr, cg, cb);
-            if (threadIdx.x == 0) done[gt] = 0u;
+            if (threadIdx.x == 0) {
+                done[gt] = 0u;
+                // The tile's bit in its row's open-tile mask, and the view's count.
+                atomicOr(&openm[views[lo].tile_base + ty * ((v.tiles_x + 31u) >> 5) + (tx >> 5)], 1u << (tx & 31u));
+                atomicAdd(&vopen[lo], 1u);
+            }
             return;
         }
         if (threadIdx.x == 0) done[gt] = 1u;
@@ -252,20 +258,21 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
 template <int MODE>
 void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles, uint32_t total_sbs,
                  const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
-                 uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate, int batch) {
+                 uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate,
+                 uint32_t *openm, uint32_t *vopen, int batch) {
     const dim3 grid(blend_grid(total_sbs)), block(BLEND_THREADS);
     switch (batch) {
     case 64:
         hipLaunchKernelGGL((k_blend<64, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     case 128:
         hipLaunchKernelGGL((k_blend<128, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     default:
         hipLaunchKernelGGL((k_blend<256, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, total_sbs, counters, pix_T, pix_last, done, pstate);
+                           rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     }
 }
@@ -275,25 +282,25 @@ void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                         uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
-                        int batch) {
+                        uint32_t *openm, uint32_t *vopen, int batch) {
     if (total_tiles == 0) return hipSuccess;
     float4 *ps = reinterpret_cast<float4 *>(pstate);
     switch (mode) {
     case BM_DIFF:
         launch_mode<BM_DIFF>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
-                             done, ps, batch);
+                             done, ps, openm, vopen, batch);
         break;
     case BM_FIRST:
         launch_mode<BM_FIRST>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
-                              pix_last, done, ps, batch);
+                              pix_last, done, ps, openm, vopen, batch);
         break;
     case BM_SECOND:
         launch_mode<BM_SECOND>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
-                               pix_last, done, ps, batch);
+                               pix_last, done, ps, openm, vopen, batch);
         break;
     default:
         launch_mode<BM_RGB>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
-                            done, ps, batch);
+                            done, ps, openm, vopen, batch);
         break;
     }
     return hipGetLastError();

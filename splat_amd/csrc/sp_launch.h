@@ -80,8 +80,9 @@ uint32_t emit_blocks(uint32_t n);
 // One (global tile id, record index) pair per overlapped 16x16 tile of each
 // view's depth-ordered splats, views in order, for emission phase `phase`
 // (EMIT_FIRST: the nearest K1 splats of each view, every tile; EMIT_SECOND:
-// the rest, only tiles with ip's row prefix counting them unfinished; vopen[v]
-// = 0 skips view v). FIRST clears vopen (when given) for the row prefixes.
+// the rest, only the tiles whose bits are set in openm, the per-row open-tile
+// masks of the first blend; vopen[v] = 0 skips view v). FIRST clears openm and
+// vopen (when given) for that blend.
 // bcnt, bcnt_all: nviews rows of emit_blocks(n) block counts; vtot: nviews
 // entries. Sets counters[CNT_PAIRS] (FIRST) or counters[CNT_PAIRS2]
 // (SECOND), saturated at 0xFFFFFFFF, and counters[CNT_OVERFLOW] when above pair_cap. FIRST
@@ -97,7 +98,7 @@ uint32_t emit_blocks(uint32_t n);
 // record's first slot.
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
+                       EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
                        uint32_t *counters);
@@ -114,10 +115,6 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
                             const uint32_t *sval, uint32_t *segc, uint32_t *sbtile, uint32_t *tcount, uint32_t *trow,
                             uint32_t *tile_start, uint32_t *pkey, uint32_t *pval, uint64_t pair_cap,
                             const uint32_t *counters);
-// Inclusive per-row prefix counts of unfinished tiles (done[t] == 0) of every
-// view, and each view's unfinished-tile count vopen[v] (read by SECOND).
-hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
-                             uint32_t *ip, uint32_t *vopen);
 // tile_start[0..num_tiles] from the sorted pair keys (tile or superblock id =
 // key & key_mask); *count pairs.
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask, const uint32_t *count,
@@ -128,12 +125,15 @@ hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask,
 // writes the float image (ViewParams::fimg) and the per-pixel final transmittance / last
 // contributing list position (pix_T / pix_last, total_tiles * 256 each).
 // BM_FIRST sets done[t] per tile and keeps unfinished tiles' pixel state in
-// pstate (total_tiles * 256 * 4 floats); BM_SECOND resumes from it.
+// pstate (total_tiles * 256 * 4 floats); BM_SECOND resumes from it. BM_FIRST
+// also sets each unfinished tile's bit in openm (row ty of view v in the words
+// from tile_base + ty * ceil(tiles_x / 32), cleared by the first emission) and
+// counts the view's unfinished tiles in vopen[v].
 // batch = records staged through LDS per step (64, 128 or 256).
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                         uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
-                        int batch);
+                        uint32_t *openm, uint32_t *vopen, int batch);
 
 // ---- diff.hip: adjoint of the blend and of the projection ----
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission

@@ -25,9 +25,10 @@
 // T < 1e-4) within the first few percent of its depth-ordered list, so the
 // plain path emits in two phases: FIRST emits the pairs of each view's
 // nearest K1 splats (K1 = max(kmin, vis / 2^shift), capped at vis) for every
-// tile; the blend then marks the tiles it finished. SECOND emits the pairs of
-// the remaining splats for the unfinished tiles only, counted in O(rows) per
-// splat from a per-row prefix count of unfinished tiles. A tile's SECOND list
+// tile; the blend then marks the tiles it finished and sets the bits of the
+// others in per-row open-tile masks. SECOND emits the pairs of the remaining
+// splats for the unfinished tiles only, counted per splat with one masked
+// popcount per row (and word) of its rectangle. A tile's SECOND list
 // continues its FIRST list in the same (depth, index) order, so a pixel sees
 // exactly the sequence of the one-phase path and the image bits are the same.
 // FIRST also scans each view's unsplit pair count into view_pairs[v]
@@ -133,28 +134,58 @@ __device__ __forceinline__ uint32_t first_count(uint32_t vis, EmitSplit sp) {
     return min(vis, max(sp.kmin, f));
 }
 
-// Unfinished tiles of row `row_base` (global id of the row's first tile) in
-// columns [0, x]: the inclusive row prefix.
-__device__ __forceinline__ uint32_t row_incl(const uint32_t *__restrict__ ip, uint32_t row_base, int x) {
-    return x < 0 ? 0u : ip[row_base + (uint32_t)x];
+// Unfinished tiles as bits: row ty of view v holds its tiles' bits in the
+// words [tile_base + ty * W, tile_base + (ty + 1) * W), W = ceil(tiles_x / 32)
+// (bit x & 31 of word x >> 5), set by the first blend for each tile it leaves
+// open. A view needs at most its tile count of words.
+__device__ __forceinline__ uint32_t mask_words(uint32_t tiles_x) { return (tiles_x + 31u) >> 5; }
+
+// Unfinished tiles among columns [x0, x1) (x0 < x1) of one row's mask words.
+__device__ __forceinline__ uint32_t row_open(const uint32_t *__restrict__ rm, uint32_t x0, uint32_t x1) {
+    uint32_t c = 0;
+    for (uint32_t w = x0 >> 5; w <= (x1 - 1u) >> 5; ++w) {
+        uint32_t m = rm[w];
+        const uint32_t lo = w << 5;
+        if (x0 > lo) m &= ~0u << (x0 - lo);
+        if (x1 < lo + 32u) m &= (1u << (x1 - lo)) - 1u;
+        c += (uint32_t)__popc(m);
+    }
+    return c;
+}
+
+// Column of the rem-th (from 0) unfinished tile among columns [x0, x1) of a row
+// that holds more than rem of them.
+__device__ __forceinline__ uint32_t row_nth_open(const uint32_t *__restrict__ rm, uint32_t x0, uint32_t x1,
+                                                 uint32_t rem) {
+    for (uint32_t w = x0 >> 5; w <= (x1 - 1u) >> 5; ++w) {
+        uint32_t m = rm[w];
+        const uint32_t lo = w << 5;
+        if (x0 > lo) m &= ~0u << (x0 - lo);
+        if (x1 < lo + 32u) m &= (1u << (x1 - lo)) - 1u;
+        const uint32_t c = (uint32_t)__popc(m);
+        if (rem < c) {
+            for (uint32_t i = 0; i < rem; ++i) m &= m - 1u;
+            return lo + (uint32_t)__builtin_ctz(m);
+        }
+        rem -= c;
+    }
+    return x1 - 1u;  // not reached for a valid rem
 }
 
 // Unfinished tiles of the rectangle of r (SECOND phase).
-__device__ __forceinline__ uint32_t open_in_rect(const uint32_t *__restrict__ ip, const Rect &r, uint32_t tile_base,
-                                                 uint32_t tiles_x) {
+__device__ __forceinline__ uint32_t open_in_rect(const uint32_t *__restrict__ openm, const Rect &r,
+                                                 uint32_t tile_base, uint32_t tiles_x) {
+    const uint32_t W = mask_words(tiles_x);
     uint32_t c = 0;
-    for (uint32_t ty = r.ty0; ty < r.ty1; ++ty) {
-        const uint32_t rb = tile_base + ty * tiles_x;
-        c += row_incl(ip, rb, (int)r.tx1 - 1) - row_incl(ip, rb, (int)r.tx0 - 1);
-    }
+    for (uint32_t ty = r.ty0; ty < r.ty1; ++ty) c += row_open(openm + tile_base + ty * W, r.tx0, r.tx1);
     return c;
 }
 
 template <int PHASE>
 __device__ __forceinline__ uint32_t phase_count(const Rect &r, uint32_t j, uint32_t k1,
-                                                const uint32_t *__restrict__ ip, uint32_t tile_base,
+                                                const uint32_t *__restrict__ openm, uint32_t tile_base,
                                                 uint32_t tiles_x) {
-    if (PHASE == EMIT_SECOND) return (j >= k1 && r.cnt) ? open_in_rect(ip, r, tile_base, tiles_x) : 0u;
+    if (PHASE == EMIT_SECOND) return (j >= k1 && r.cnt) ? open_in_rect(openm, r, tile_base, tiles_x) : 0u;
     return j < k1 ? r.cnt : 0u;
 }
 
@@ -163,7 +194,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
                                                              uint32_t bpv, const uint32_t *__restrict__ vis_count,
                                                              const uint2 *__restrict__ sbox, uint2 *__restrict__ box,
                                                              const ViewParams *__restrict__ views, EmitSplit split,
-                                                             const uint32_t *__restrict__ ip,
+                                                             uint32_t *__restrict__ openm,
                                                              uint32_t *__restrict__ bcnt,
                                                              uint32_t *__restrict__ bcnt_all,
                                                              uint32_t *__restrict__ bsb,
@@ -173,7 +204,13 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
         if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
         return;
     }
-    if (PHASE == EMIT_FIRST && vopen && local == 0 && threadIdx.x == 0) vopen[v] = 0;  // row prefixes add to it
+    if (PHASE == EMIT_FIRST && vopen) {
+        // The first blend sets the bits of the tiles it leaves open and counts them.
+        if (local == 0 && threadIdx.x == 0) vopen[v] = 0;
+        const uint32_t nw = views[v].ctx.tiles_y * mask_words(views[v].ctx.tiles_x);
+        for (uint32_t i = local * EMIT_THREADS + threadIdx.x; i < nw; i += bpv * EMIT_THREADS)
+            openm[views[v].tile_base + i] = 0u;
+    }
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);  // dval holds n_splats entries per view
     const uint32_t k1 = first_count(nvis, split);
@@ -194,7 +231,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
     } else {
         r = splat_rect<false>(dval, box, vbase, j, nvis, tiles_x, tile_base);
     }
-    const uint32_t c = phase_count<PHASE>(r, j, k1, ip, tile_base, tiles_x);
+    const uint32_t c = phase_count<PHASE>(r, j, k1, openm, tile_base, tiles_x);
     if (PHASE == EMIT_SECOND) {
         const uint32_t v1[1] = {c};
         uint32_t t1[1];
@@ -338,7 +375,8 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
                                                                    uint32_t bpv, const uint32_t *__restrict__ vis_count,
                                                                    const uint2 *__restrict__ box,
                                                                    const ViewParams *__restrict__ views,
-                                                                   EmitSplit split, const uint32_t *__restrict__ ip,
+                                                                   EmitSplit split,
+                                                                   const uint32_t *__restrict__ openm,
                                                                    const uint32_t *__restrict__ boff,
                                                                    const uint32_t *__restrict__ vbase_pairs,
                                                                    uint32_t *__restrict__ pkey,
@@ -367,7 +405,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
     const Rect r = splat_rect(dval, box, vbase, j, nvis, tiles_x, tile_base);
     uint32_t total;
     const uint32_t lscan =
-        block_exclusive_scan<EMIT_THREADS>(phase_count<EMIT_SECOND>(r, j, k1, ip, tile_base, tiles_x), &total);
+        block_exclusive_scan<EMIT_THREADS>(phase_count<EMIT_SECOND>(r, j, k1, openm, tile_base, tiles_x), &total);
     s_pre[threadIdx.x] = lscan;
     s_org[threadIdx.x] = r.org;
     s_rw[threadIdx.x] = r.rw;
@@ -388,54 +426,20 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_expand_open(const uint32_
         const uint32_t rwl = s_rw[lo];
         // loc-th unfinished tile of the rectangle, rows top to bottom.
         const uint32_t org = s_org[lo] - tile_base;
-        const uint32_t ty0 = org / tiles_x, tx0 = org % tiles_x;
+        const uint32_t ty0 = org / tiles_x, tx0 = org % tiles_x, W = mask_words(tiles_x);
         uint32_t rem = loc, ty = ty0;
-        uint32_t rb = tile_base + ty * tiles_x, before = row_incl(ip, rb, (int)tx0 - 1);
-        for (uint32_t k = 0; k < s_rows[lo]; ++k, ++ty) {
-            rb = tile_base + ty * tiles_x;
-            before = row_incl(ip, rb, (int)tx0 - 1);
-            const uint32_t c = row_incl(ip, rb, (int)(tx0 + rwl) - 1) - before;
+        for (uint32_t k = 0; k + 1 < s_rows[lo]; ++k, ++ty) {
+            const uint32_t c = row_open(openm + tile_base + ty * W, tx0, tx0 + rwl);
             if (rem < c) break;
             rem -= c;
         }
-        // Smallest x in [tx0, tx0 + rwl) with ip(x) - before > rem.
-        uint32_t xl = tx0, xh = tx0 + rwl - 1;
-        while (xl < xh) {
-            const uint32_t mid = (xl + xh) >> 1;
-            if (ip[rb + mid] - before > rem)
-                xh = mid;
-            else
-                xl = mid + 1;
-        }
+        const uint32_t xl = row_nth_open(openm + tile_base + ty * W, tx0, tx0 + rwl, rem);
+        const uint32_t rb = tile_base + ty * tiles_x;
         const uint64_t pos = base + q;
         if (pos < pair_cap) {
             pkey[pos] = rb + xl;
             pval[pos] = s_id[lo];
         }
-    }
-}
-
-// Inclusive per-row prefix counts of unfinished tiles: one wave per tile
-// row (lanes over 64 columns at a time), blockIdx.y = view. Each row also
-// adds its count to vopen[v] (cleared by the first phase's count pass), so
-// that the second phase passes over views the first phase finished.
-__global__ __launch_bounds__(256) void k_row_prefix(const ViewParams *__restrict__ views,
-                                                    const uint32_t *__restrict__ done, uint32_t *__restrict__ ip,
-                                                    uint32_t *__restrict__ vopen) {
-    const ViewParams &vp = views[blockIdx.y];
-    const uint32_t tx_n = vp.ctx.tiles_x, ty_n = vp.ctx.tiles_y;
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    for (uint32_t ty = blockIdx.x * 4 + w; ty < ty_n; ty += gridDim.x * 4) {
-        const uint32_t rb = vp.tile_base + ty * tx_n;
-        uint32_t carry = 0;
-        for (uint32_t x0 = 0; x0 < tx_n; x0 += 64) {
-            const uint32_t x = x0 + lane;
-            const uint32_t open = (x < tx_n && !done[rb + x]) ? 1u : 0u;
-            const uint32_t inc = wave_inclusive_scan(open);
-            if (x < tx_n) ip[rb + x] = carry + inc;
-            carry += (uint32_t)__shfl((int)inc, 63);
-        }
-        if (lane == 0 && carry) atomicAdd(&vopen[blockIdx.y], carry);
     }
 }
 
@@ -763,14 +767,14 @@ template <int PHASE>
 hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                       const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                       EmitSplit split,
-                      const uint32_t *ip, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
+                      uint32_t *openm, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *bsb, uint32_t *vsb, uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
                       uint32_t *powner, uint32_t *pbase, uint64_t pair_cap, uint32_t count_slot,
                       uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, sbox, box, views, split, ip, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
+                       vis_count, sbox, box, views, split, openm, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
     hipError_t e = hipGetLastError();
     // Per view: block offsets and the view's pair total (and, in FIRST, the
     // unsplit pair count of every view, RenderStats.overlaps, and the binned
@@ -800,7 +804,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
         }
     } else if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                           vis_count, box, views, split, ip, bcnt, vtot, pkey, pval, pair_cap, counters);
+                           vis_count, box, views, split, openm, bcnt, vtot, pkey, pval, pair_cap, counters);
     }
     return hipGetLastError();
 }
@@ -811,16 +815,16 @@ uint32_t emit_blocks(uint32_t n) { return n ? (n + EMIT_THREADS - 1) / EMIT_THRE
 
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
-                       EmitSplit split, const uint32_t *ip, uint32_t *vopen, uint32_t *bcnt,
+                       EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
                        uint32_t *counters) {
     if (phase == EMIT_SECOND)
-        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt,
+        return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt,
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, nullptr,
                                        nullptr, pair_cap, CNT_PAIRS2, counters);
     if (!bsb || !vsb) return hipErrorInvalidValue;  // the first phase is always binned
-    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, ip, bcnt, bcnt_all,
+    return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt, bcnt_all,
                                   vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pbase, pair_cap, CNT_PAIRS,
                                   counters);
 }
@@ -851,13 +855,6 @@ hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nvi
 }
 
 uint32_t bin_scan_rows(uint32_t total_tiles) { return (total_tiles + BIN_SCAN_ROW - 1) / BIN_SCAN_ROW; }
-
-hipError_t launch_row_prefix(hipStream_t s, const ViewParams *views, uint32_t nviews, const uint32_t *done,
-                             uint32_t *ip, uint32_t *vopen) {
-    if (nviews == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_row_prefix, dim3(16, nviews), dim3(256), 0, s, views, done, ip, vopen);
-    return hipGetLastError();
-}
 
 hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask, const uint32_t *count,
                          uint64_t pair_cap, uint32_t num_tiles, uint32_t *tile_start) {
