@@ -10,11 +10,8 @@
 // pixel box misses the wave's quadrant, or whose falloff cannot reach the
 // splat's alpha threshold at any in-box pixel of the quadrant, is never
 // visited (the wave ballots the reach bits of 64 staged records at a time and
-// walks only the set bits, in list order); one whose falloff stays below the
-// threshold at every live in-box pixel is skipped after the falloff test
-// (those pixels are masked by the reference too). A pixel stops at
-// T < T_MIN; a wave stops when its quadrant has no live pixel, the tile when
-// no pixel is live.
+// walks only the set bits, in list order). A pixel stops at T < T_MIN; a wave
+// stops when its quadrant has no live pixel, the tile when no pixel is live.
 //
 // Per-pixel arithmetic is raster_tile's (src/render/cpu.rs:609-719) in the
 // same operation order, with pixel offsets built from the 16-aligned tile
@@ -46,17 +43,15 @@ namespace {
 // visit needs three operations for it. A column or row outside the record's
 // pixel box stores a NaN in place of qx or qy: the power is then NaN, which
 // fails `pw <= 0`, so the pixel is masked as the reference masks it by the box
-// (whatever the opacity), and the threshold compare skips it too.
-// 304 bytes, an odd number of 16-byte units, so the 16-byte writes of eight
+// (whatever the opacity).
+// 272 bytes, an odd number of 16-byte units, so the 16-byte writes of eight
 // consecutive staging lanes fall in distinct LDS bank groups.
 struct StagedRec {
     float2 col[TILE];  // (dx, qx = (dx * dx) * (-a/2)) per tile column
     float2 row[TILE];  // (qy = ((-c/2) * dy) * dy, -b * dy) per tile row
     float4 orgb;       // opacity, colour
-    float thr, pad0, pad1, pad2;
-    float4 pad3;
 };
-static_assert(sizeof(StagedRec) == 304, "staged record size");
+static_assert(sizeof(StagedRec) == 272, "staged record size");
 
 // Stages part `part` of P of one record for the tile at (ox, oy): its
 // columns and rows [part * 16/P, (part + 1) * 16/P) and the reach bits of its
@@ -101,11 +96,9 @@ __device__ __forceinline__ void stage_part(StagedRec &d, uint8_t *qb, const uint
         *reinterpret_cast<float4 *>(&d.row[r]) = make_float4((r >= ly0 && r < ly1) ? qa : qnan, nb2 * dya,
                                                              (r + 1 >= ly0 && r + 1 < ly1) ? qb2 : qnan, nb2 * dyb);
     }
-    if (part == 0) {
+    if (part == 0)
         d.orgb = make_float4(__uint_as_float(r1.y), __uint_as_float(r1.z), __uint_as_float(r1.w),
                              __uint_as_float(r2.x));
-        d.thr = thr;
-    }
     const float ia = 1.0f / (2.0f * na), ic = 1.0f / (2.0f * nc);
     uint32_t bits = 0;
 #pragma unroll
@@ -192,28 +185,23 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
                 const float2 cv = sr.col[lx];
                 const float2 rv = sr.row[ly];
                 const float4 c4 = sr.orgb;
-                const float thr = sr.thr;
-                // The record's five wave-uniform words are read at once from one
+                // The record's four wave-uniform words are read at once from one
                 // address register (the compiler would otherwise sink each read
                 // to its first use and rebuild the address for each).
-                asm volatile("" ::"v"(c4.x), "v"(c4.y), "v"(c4.z), "v"(c4.w), "v"(thr));
+                asm volatile("" ::"v"(c4.x), "v"(c4.y), "v"(c4.z), "v"(c4.w));
                 const float pw = (cv.y + rv.x) + rv.y * cv.x;
-                // Lanes below the splat's threshold have alpha < ALPHA_MIN (the
-                // reference masks them); a wave with none left skips the rest.
-                if ((lm & lane_mask(pw >= thr)) != 0ull) {
-                    const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
-                    const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
-                    if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
-                        const float wgt = alpha * T;
-                        cr = cr + c4.y * wgt;
-                        cg = cg + c4.z * wgt;
-                        cb = cb + c4.w * wgt;
-                        T = T * (1.0f - alpha);
-                        if (MODE == BM_DIFF) last = (b - start) + k + 1;
-                    }
-                    lm = live_mask();
-                    if (lm == 0ull) reach = 0ull;  // every pixel of the quadrant saturated
+                const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
+                const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
+                    const float wgt = alpha * T;
+                    cr = cr + c4.y * wgt;
+                    cg = cg + c4.z * wgt;
+                    cb = cb + c4.w * wgt;
+                    T = T * (1.0f - alpha);
+                    if (MODE == BM_DIFF) last = (b - start) + k + 1;
                 }
+                lm = live_mask();
+                if (lm == 0ull) reach = 0ull;  // every pixel of the quadrant saturated
             }
         }
         if (__syncthreads_count(live()) == 0) {
