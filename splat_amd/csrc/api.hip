@@ -143,6 +143,10 @@ struct sp_handle {
     float *d_soa = nullptr;
     uint16_t *d_soa16 = nullptr;
     uint64_t soa_bytes = 0, soa16_bytes = 0;
+    // The scene may give a splat an infinite colour (launch_prep's flag): its
+    // frames blend through the instance that reproduces the reference's
+    // NaN channels beside such splats.
+    bool extreme_colour = false;
     // Per-view arrays, view_cap views of n entries each.
     uint32_t view_cap = 0;
     uint4 *d_rec = nullptr;
@@ -624,7 +628,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         const int bm = split ? sp::BM_FIRST : (mode == BL_DIFF ? sp::BM_DIFF : sp::BM_RGB);
         HIP_TRY(sp::launch_blend(s, bm, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start, h->d_pval[pi],
                                  h->d_rec, h->d_counters, h->d_pixT, h->d_pixlast, h->d_done, h->d_pstate,
-                                 split ? h->d_openm : nullptr, split ? vopen : nullptr, h->blend_batch));
+                                 split ? h->d_openm : nullptr, split ? vopen : nullptr, h->blend_batch,
+                                 h->extreme_colour));
     }
     HIP_TRY(mark(3));
     if (after_first) {
@@ -648,7 +653,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
         HIP_TRY(sp::launch_blend(s, sp::BM_SECOND, h->d_views, nviews, total_tiles, total_sbs, h->d_tile_start,
                                  h->d_pval[pi],
                                  h->d_rec, h->d_counters, nullptr, nullptr, h->d_done, h->d_pstate, nullptr,
-                                 nullptr, h->blend_batch));
+                                 nullptr, h->blend_batch, h->extreme_colour));
     HIP_TRY(mark(6));
     // Counters, visible counts and pair counts (view_cap apart) in one copy.
     HIP_TRY(hipMemcpyAsync(h->h_counters, h->d_counters, (sp::CNT_SLOTS + h->view_cap + nviews) * 4,
@@ -842,7 +847,7 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     const uint64_t geo = f32 ? 6ull : 7ull;  // cov3d, or scale + quaternion
     const uint64_t per = 3 + geo + 1 + 1 + 3 + 3ull * rest;
     float *tmp = nullptr;
-    rc = dalloc(h, (void **)&tmp, per * n * 4);
+    rc = dalloc(h, (void **)&tmp, (per * n + 1) * 4);
     if (rc) {
         for (auto &th : pool) th.join();
         return rc;
@@ -868,13 +873,19 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     in.k = put(nullptr, n);
     in.col = put(scene->colors, 3ull * n);
     in.sh = put(rest ? scene->sh_rest : nullptr, 3ull * rest * n);
+    // One more word after the staged arrays: the extreme-colour flag.
+    uint32_t *d_flag = reinterpret_cast<uint32_t *>(p);
+    if (!rc && hipMemsetAsync(d_flag, 0, 4, h->stream) != hipSuccess) rc = SP_E_DEVICE;
     for (auto &th : pool) th.join();
     if (!rc && hipMemcpyAsync(dk, kv.data(), 4ull * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = SP_E_DEVICE;
     hipError_t e = rc ? hipErrorUnknown
-                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, S, h->d_soa, h->d_soa16);
+                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, S, h->d_soa, h->d_soa16, d_flag);
+    uint32_t flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    dfree(h, tmp, per * n * 4);
+    h->extreme_colour = flag != 0;
+    dfree(h, tmp, (per * n + 1) * 4);
     if (rc || e != hipSuccess) return fail(SP_E_DEVICE, std::string("scene upload failed: ") + hipGetErrorString(e));
     return SP_OK;
 }

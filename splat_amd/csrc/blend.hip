@@ -31,6 +31,10 @@
 //              other tile keeps (T, C) per pixel in pstate for BM_SECOND.
 //   BM_SECOND  the continuation list of the tiles not done: resumes from
 //              pstate and writes the RGB8 output.
+// EXACT instances serve scenes that may give a splat an infinite colour
+// (flagged at upload): the reference's 4-lane groups then make NaN channels
+// beside the lanes that blend, which they reproduce; other scenes never reach
+// that code.
 #include "sp_device.h"
 #include "sp_launch.h"
 
@@ -61,7 +65,7 @@ static_assert(sizeof(StagedRec) == 272, "staged record size");
 // cx0 = (ox + 0.5) - x, dx = lane + (4*group + cx0) per column,
 // dy = ((oy + r) + 0.5) - y per row. Frame coordinates are below 2^16, so
 // (oy + r0) + (j + 0.5) is exact and equals (oy + r) + 0.5 for r = r0 + j.
-template <int P>
+template <int P, bool EXACT>
 __device__ __forceinline__ void stage_part(StagedRec &d, uint8_t *qb, const uint4 &r0, const uint4 &r1,
                                            const uint4 &r2, uint32_t ox, uint32_t oy, uint32_t part) {
     constexpr int NC = (int)TILE / P, NQ = 4 / P;
@@ -99,18 +103,28 @@ __device__ __forceinline__ void stage_part(StagedRec &d, uint8_t *qb, const uint
     if (part == 0)
         d.orgb = make_float4(__uint_as_float(r1.y), __uint_as_float(r1.z), __uint_as_float(r1.w),
                              __uint_as_float(r2.x));
+    // Colours are >= 0 and never NaN. In the EXACT blend a record with an
+    // infinite colour marks every quadrant its box overlaps with a second bit
+    // (value 3 instead of 1), whether or not its falloff reaches it: there the
+    // reference's masked lanes turn NaN, and the blend walks a 64-record group
+    // that holds such a record on its exact path.
+    const bool inf_col = EXACT && (__builtin_isinf(__uint_as_float(r1.z)) || __builtin_isinf(__uint_as_float(r1.w)) ||
+                                   __builtin_isinf(__uint_as_float(r2.x)));
     const float ia = 1.0f / (2.0f * na), ic = 1.0f / (2.0f * nc);
     uint32_t bits = 0;
 #pragma unroll
-    for (int k = 0; k < NQ; ++k)
-        if (quadrant_reaches(sx, sy, na, nb2, nc, ia, ic, thr, bx0, by0, bx1, by1, ox, oy, part * NQ + k))
-            bits |= 1u << (8 * k);
+    for (int k = 0; k < NQ; ++k) {
+        const uint32_t q = part * NQ + k;
+        if (inf_col ? quadrant_overlaps(bx0, by0, bx1, by1, ox, oy, q)
+                    : quadrant_reaches(sx, sy, na, nb2, nc, ia, ic, thr, bx0, by0, bx1, by1, ox, oy, q))
+            bits |= (inf_col ? 3u : 1u) << (8 * k);
+    }
     if constexpr (NQ == 4) *reinterpret_cast<uint32_t *>(qb) = bits;
     else if constexpr (NQ == 2) *reinterpret_cast<uint16_t *>(qb + 2 * part) = (uint16_t)bits;
     else qb[part] = (uint8_t)bits;
 }
 
-template <int BATCH, int MODE>
+template <int BATCH, int MODE, bool EXACT>
 __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(BATCH == 64 ? 8 : 1))) void k_blend(const ViewParams *__restrict__ views, uint32_t nviews,
                                                          uint32_t total_tiles, const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ sval,
@@ -162,7 +176,7 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
             const uint32_t id = sval[idx];
             const uint4 *r = rec + 3 * (uint64_t)id;
             const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
-            stage_part<P>(s_st[rr], s_qb[rr], r0, r1, r2, ox, oy, threadIdx.x % P);
+            stage_part<P, EXACT>(s_st[rr], s_qb[rr], r0, r1, r2, ox, oy, threadIdx.x % P);
         }
         __syncthreads();
         const uint32_t nb = __builtin_amdgcn_readfirstlane(min(end - b, (uint32_t)BATCH));
@@ -175,33 +189,119 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
         for (uint32_t g0 = 0; g0 < nb && lm != 0ull; g0 += 64) {
             const uint32_t qm = g0 + lane_i < nb ? s_qb[g0 + lane_i][w] : 0u;
             unsigned long long reach = lane_mask(qm != 0u);
-            while (reach) {
-                const uint32_t kb = (uint32_t)__builtin_ctzll(reach);
-                const uint32_t k = g0 + kb;
-                // Clears the visited bit: one scalar instruction instead of the
-                // three of a 64-bit reach &= reach - 1.
-                asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kb));
-                const StagedRec &sr = s_st[k];
-                const float2 cv = sr.col[lx];
-                const float2 rv = sr.row[ly];
-                const float4 c4 = sr.orgb;
-                // The record's four wave-uniform words are read at once from one
-                // address register (the compiler would otherwise sink each read
-                // to its first use and rebuild the address for each).
-                asm volatile("" ::"v"(c4.x), "v"(c4.y), "v"(c4.z), "v"(c4.w));
-                const float pw = (cv.y + rv.x) + rv.y * cv.x;
+            if (reach == 0ull) continue;
+            if (EXACT && lane_mask(qm > 1u) != 0ull) {
+                // A record of the group has an infinite colour. The reference
+                // blends 4-lane groups of a row (src/render/cpu.rs:673-690):
+                // once any lane of a group is live and inside the record's
+                // box, every lane of the group adds colour * (alpha * T) with
+                // alpha zeroed where it does not blend, and inf * 0 = NaN. The
+                // fast walk below changes only the lanes that blend, which is
+                // the same for finite colours only; this walk reproduces the
+                // groups. A staged term is NaN exactly outside the box.
+                while (reach) {
+                    const uint32_t kb = (uint32_t)__builtin_ctzll(reach);
+                    asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kb));
+                    const StagedRec &sr = s_st[g0 + kb];
+                    const float2 cv = sr.col[lx], rv = sr.row[ly];
+                    const float4 c4 = sr.orgb;
+                    const float pw = (cv.y + rv.x) + rv.y * cv.x;
+                    const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
+                    const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                    const uint64_t act = lm & lane_mask(!__builtin_isnan(cv.y) && !__builtin_isnan(rv.x));
+                    // Lanes whose 4-lane group (a nibble of the 8x8 quadrant's
+                    // lane mask: 4 consecutive columns of a row, 4-aligned in
+                    // the tile) holds an active lane.
+                    const uint64_t grp = ((act | (act >> 1) | (act >> 2) | (act >> 3)) & 0x1111111111111111ull) * 0xFull;
+                    if (__builtin_amdgcn_inverse_ballot_w64(grp)) {
+                        const float a = __builtin_amdgcn_inverse_ballot_w64(upd) ? alpha : 0.0f;
+                        const float wgt = a * T;
+                        cr = cr + c4.y * wgt;
+                        cg = cg + c4.z * wgt;
+                        cb = cb + c4.w * wgt;
+                        T = T * (1.0f - a);
+                        if (MODE == BM_DIFF && __builtin_amdgcn_inverse_ballot_w64(upd))
+                            last = (b - start) + g0 + kb + 1;
+                    }
+                    lm = live_mask();
+                    if (lm == 0ull) break;
+                }
+                continue;
+            }
+            // The visits are software-pipelined over two register sets: each
+            // reads the next set record's terms from LDS before its own
+            // arithmetic, so the LDS latency overlaps the previous visit's
+            // dependent chain, and the sets alternate (no copies). The scalar
+            // work per visit is kept small, since the CU's scalar unit issues
+            // one instruction per SIMD turn while the SIMD issues up to two
+            // VALU: the next record's index is s_ff1 clamped to 63 (s_ff1 of
+            // an empty mask is -1; record 63 of the group is an in-bounds
+            // read whose values are then never used), the update is
+            // branch-free (a lane that must not update blends alpha = 0,
+            // which leaves its C and T unchanged: C + c * (0 * T) = C for a
+            // finite colour and T * (1 - 0) = T exactly), and the
+            // per-lane LDS byte offsets of the group's terms are set once per
+            // group. 43 VALU and 9 scalar instructions per visit.
+            const char *gbase = reinterpret_cast<const char *>(&s_st[g0]);
+            const char *pcol = gbase + lx * sizeof(float2);
+            const char *prow = gbase + offsetof(StagedRec, row) + ly * sizeof(float2);
+            // A lane-varying register for the uniform offset too, so each read's
+            // address is one vector add (a scalar add and a move otherwise).
+            uint32_t orgb_off = (uint32_t)offsetof(StagedRec, orgb);
+            asm("" : "+v"(orgb_off));
+            const char *porgb = gbase + orgb_off;
+            const auto load = [&](uint32_t k, float2 &cv, float2 &rv, float4 &c4) {
+                const uint32_t off = k * (uint32_t)sizeof(StagedRec);
+                cv = *reinterpret_cast<const float2 *>(pcol + off);
+                rv = *reinterpret_cast<const float2 *>(prow + off);
+                c4 = *reinterpret_cast<const float4 *>(porgb + off);
+            };
+            const auto first_set = [](uint64_t m) {
+                uint32_t k;
+                asm("s_ff1_i32_b64 %0, %1" : "=s"(k) : "s"(m));
+                return min(k, 63u);
+            };
+            uint32_t kA = first_set(reach), kB = 0;
+            // Clears the visited bit: one scalar instruction instead of the
+            // three of a 64-bit reach &= reach - 1.
+            asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kA));
+            float2 cvA, rvA, cvB, rvB;
+            float4 c4A, c4B;
+            load(kA, cvA, rvA, c4A);
+            // Visits record kc (terms cv, rv, c4) after reading the terms of
+            // the next set record kn into the other set. Returns whether the
+            // walk goes on: a set record is left and a pixel of the quadrant
+            // is live; then kn's bit is cleared.
+            const auto visit = [&](const float2 &cv, const float2 &rv, const float4 &c4, uint32_t kc, float2 &cvn,
+                                   float2 &rvn, float4 &c4n, uint32_t &kn) -> bool {
+                kn = first_set(reach);
+                load(kn, cvn, rvn, c4n);
+                // The arithmetic starts from values this empty asm passes on,
+                // after the reads above (its memory clobber keeps them ahead of
+                // it): the compiler would otherwise issue the next record's
+                // reads late in the visit, or sink them past the exit test.
+                float cx = cv.x, cy = cv.y, rx = rv.x, ry = rv.y;
+                asm volatile("" : "+v"(cx), "+v"(cy), "+v"(rx), "+v"(ry) : : "memory");
+                const float pw = (cy + rx) + ry * cx;
                 const float alpha = fminf(exp_neg_times(pw, c4.x), ALPHA_MAX);
                 const uint64_t upd = lm & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
-                if (__builtin_amdgcn_inverse_ballot_w64(upd)) {
-                    const float wgt = alpha * T;
-                    cr = cr + c4.y * wgt;
-                    cg = cg + c4.z * wgt;
-                    cb = cb + c4.w * wgt;
-                    T = T * (1.0f - alpha);
-                    if (MODE == BM_DIFF) last = (b - start) + k + 1;
-                }
+                const float a = __builtin_amdgcn_inverse_ballot_w64(upd) ? alpha : 0.0f;
+                const float wgt = a * T;
+                cr = cr + c4.y * wgt;
+                cg = cg + c4.z * wgt;
+                cb = cb + c4.w * wgt;
+                T = T * (1.0f - a);
+                if (MODE == BM_DIFF && __builtin_amdgcn_inverse_ballot_w64(upd)) last = (b - start) + g0 + kc + 1;
                 lm = live_mask();
-                if (lm == 0ull) reach = 0ull;  // every pixel of the quadrant saturated
+                // One test ends the walk: a saturated quadrant empties the
+                // reach mask (two scalar instructions; the compiler would
+                // combine the two tests through five).
+                asm("s_cmp_eq_u64 %1, 0\n\ts_cselect_b64 %0, 0, %0" : "+s"(reach) : "s"(lm) : "scc");
+                if (reach == 0ull) return false;
+                asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kn));
+                return true;
+            };
+            while (visit(cvA, rvA, c4A, kA, cvB, rvB, c4B, kB) && visit(cvB, rvB, c4B, kB, cvA, rvA, c4A, kA)) {
             }
         }
         if (__syncthreads_count(live()) == 0) {
@@ -243,7 +343,7 @@ __global__ __launch_bounds__(BLEND_THREADS) __attribute__((amdgpu_waves_per_eu(B
     }
 }
 
-template <int MODE>
+template <int MODE, bool EXACT>
 void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles, uint32_t total_sbs,
                  const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                  uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate,
@@ -251,18 +351,31 @@ void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32
     const dim3 grid(blend_grid(total_sbs)), block(BLEND_THREADS);
     switch (batch) {
     case 64:
-        hipLaunchKernelGGL((k_blend<64, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
+        hipLaunchKernelGGL((k_blend<64, MODE, EXACT>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
                            rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     case 128:
-        hipLaunchKernelGGL((k_blend<128, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
+        hipLaunchKernelGGL((k_blend<128, MODE, EXACT>), grid, block, 0, s, views, nviews, total_tiles, tile_start,
+                           sval, rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     default:
-        hipLaunchKernelGGL((k_blend<256, MODE>), grid, block, 0, s, views, nviews, total_tiles, tile_start, sval,
-                           rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
+        hipLaunchKernelGGL((k_blend<256, MODE, EXACT>), grid, block, 0, s, views, nviews, total_tiles, tile_start,
+                           sval, rec, total_sbs, counters, pix_T, pix_last, done, pstate, openm, vopen);
         break;
     }
+}
+
+template <int MODE>
+void launch_exact(hipStream_t s, bool exact, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
+                  uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
+                  uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float4 *pstate,
+                  uint32_t *openm, uint32_t *vopen, int batch) {
+    if (exact)
+        launch_mode<MODE, true>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
+                                pix_last, done, pstate, openm, vopen, batch);
+    else
+        launch_mode<MODE, false>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
+                                 pix_last, done, pstate, openm, vopen, batch);
 }
 
 }  // namespace
@@ -270,25 +383,25 @@ void launch_mode(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                         uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
-                        uint32_t *openm, uint32_t *vopen, int batch) {
+                        uint32_t *openm, uint32_t *vopen, int batch, bool exact) {
     if (total_tiles == 0) return hipSuccess;
     float4 *ps = reinterpret_cast<float4 *>(pstate);
     switch (mode) {
     case BM_DIFF:
-        launch_mode<BM_DIFF>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
-                             done, ps, openm, vopen, batch);
+        launch_mode<BM_DIFF, false>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
+                                    pix_last, done, ps, openm, vopen, batch);
         break;
     case BM_FIRST:
-        launch_mode<BM_FIRST>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
-                              pix_last, done, ps, openm, vopen, batch);
-        break;
-    case BM_SECOND:
-        launch_mode<BM_SECOND>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
+        launch_exact<BM_FIRST>(s, exact, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
                                pix_last, done, ps, openm, vopen, batch);
         break;
+    case BM_SECOND:
+        launch_exact<BM_SECOND>(s, exact, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters,
+                                pix_T, pix_last, done, ps, openm, vopen, batch);
+        break;
     default:
-        launch_mode<BM_RGB>(s, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T, pix_last,
-                            done, ps, openm, vopen, batch);
+        launch_exact<BM_RGB>(s, exact, views, nviews, total_tiles, total_sbs, tile_start, sval, rec, counters, pix_T,
+                             pix_last, done, ps, openm, vopen, batch);
         break;
     }
     return hipGetLastError();

@@ -12,26 +12,45 @@
 // k = sqrt(2 ln(o / ALPHA_MIN)) arrives precomputed by the host, whose logf
 // the reference's bits follow.
 #include "half_conv.h"
+#include "sp_device.h"
 #include "sp_launch.h"
 
 namespace sp {
 namespace {
 
+// Sets *flag once per wave that holds a splat with an extreme colour value
+// (launch_prep).
+__device__ __forceinline__ void flag_extreme(bool bad, uint32_t *flag) {
+    if (wave_any(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+
 __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos, const float *__restrict__ cov,
                                                   const float *__restrict__ opac, const float *__restrict__ kk,
                                                   const float *__restrict__ col, const float *__restrict__ sh,
-                                                  uint32_t n, uint32_t rest, uint64_t S, float *__restrict__ soa) {
+                                                  uint32_t n, uint32_t rest, uint64_t S, float *__restrict__ soa,
+                                                  uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    bool bad = false;
+    if (i < n) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) soa[(A_POS + c) * S + i] = pos[3ull * i + c];
+        for (int c = 0; c < 3; ++c) soa[(A_POS + c) * S + i] = pos[3ull * i + c];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) soa[(A_COV + c) * S + i] = cov[6ull * i + c];
-    soa[A_OPAC * S + i] = opac[i];
-    soa[A_K * S + i] = kk[i];
+        for (int c = 0; c < 6; ++c) soa[(A_COV + c) * S + i] = cov[6ull * i + c];
+        soa[A_OPAC * S + i] = opac[i];
+        soa[A_K * S + i] = kk[i];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) soa[(A_COL + c) * S + i] = col[3ull * i + c];
-    for (uint32_t c = 0; c < 3 * rest; ++c) soa[(A_SH + c) * S + i] = sh[(uint64_t)3 * rest * i + c];
+        for (int c = 0; c < 3; ++c) {
+            const float v = col[3ull * i + c];
+            bad |= !(fabsf(v) < 1e36f);
+            soa[(A_COL + c) * S + i] = v;
+        }
+        for (uint32_t c = 0; c < 3 * rest; ++c) {
+            const float v = sh[(uint64_t)3 * rest * i + c];
+            bad |= !(fabsf(v) < 1e36f);
+            soa[(A_SH + c) * S + i] = v;
+        }
+    }
+    flag_extreme(bad, extreme);
 }
 
 template <bool BF16>
@@ -44,37 +63,50 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
                                                  const float *__restrict__ quat, const float *__restrict__ opac,
                                                  const float *__restrict__ kk, const float *__restrict__ col,
                                                  const float *__restrict__ sh, uint32_t n, uint32_t rest, uint64_t S,
-                                                 float *__restrict__ soa, uint16_t *__restrict__ soa16) {
+                                                 float *__restrict__ soa, uint16_t *__restrict__ soa16,
+                                                 uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    const float lim = BF16 ? 1e36f : 65504.0f;
+    bool bad = false;
+    if (i < n) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) soa[(L_POS + c) * S + i] = pos[3ull * i + c];
-    soa[L_OPAC * S + i] = opac[i];
-    soa[L_K * S + i] = kk[i];
+        for (int c = 0; c < 3; ++c) soa[(L_POS + c) * S + i] = pos[3ull * i + c];
+        soa[L_OPAC * S + i] = opac[i];
+        soa[L_K * S + i] = kk[i];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) soa16[(H_SCALE + c) * S + i] = to16<BF16>(scl[3ull * i + c]);
+        for (int c = 0; c < 3; ++c) soa16[(H_SCALE + c) * S + i] = to16<BF16>(scl[3ull * i + c]);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) soa16[(H_QUAT + c) * S + i] = to16<BF16>(quat[4ull * i + c]);
+        for (int c = 0; c < 4; ++c) soa16[(H_QUAT + c) * S + i] = to16<BF16>(quat[4ull * i + c]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) soa16[(H_COL + c) * S + i] = to16<BF16>(col[3ull * i + c]);
-    for (uint32_t c = 0; c < 3 * rest; ++c) soa16[(H_SH + c) * S + i] = to16<BF16>(sh[(uint64_t)3 * rest * i + c]);
+        for (int c = 0; c < 3; ++c) {
+            const float v = col[3ull * i + c];
+            bad |= !(fabsf(v) < lim);
+            soa16[(H_COL + c) * S + i] = to16<BF16>(v);
+        }
+        for (uint32_t c = 0; c < 3 * rest; ++c) {
+            const float v = sh[(uint64_t)3 * rest * i + c];
+            bad |= !(fabsf(v) < lim);
+            soa16[(H_SH + c) * S + i] = to16<BF16>(v);
+        }
+    }
+    flag_extreme(bad, extreme);
 }
 
 }  // namespace
 
 hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
-                       float *soa, uint16_t *soa16) {
+                       float *soa, uint16_t *soa16, uint32_t *extreme) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + 255) / 256;
     if (storage == SP_STORE_F32)
         hipLaunchKernelGGL(k_prep_f32, dim3(grid), dim3(256), 0, s, in.pos, in.cov, in.opac, in.k, in.col, in.sh, n,
-                           rest, S, soa);
+                           rest, S, soa, extreme);
     else if (storage == SP_STORE_F16)
         hipLaunchKernelGGL(k_prep_16<false>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, S, soa, soa16);
+                           in.col, in.sh, n, rest, S, soa, soa16, extreme);
     else
         hipLaunchKernelGGL(k_prep_16<true>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, S, soa, soa16);
+                           in.col, in.sh, n, rest, S, soa, soa16, extreme);
     return hipGetLastError();
 }
 

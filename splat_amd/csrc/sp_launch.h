@@ -48,9 +48,13 @@ struct PrepInput {
     const float *pos, *cov, *scl, *quat, *opac, *k, *col, *sh;
 };
 // Writes the structure-of-arrays scene (column stride S >= n) in `storage`'s
-// layout: soa (f32 columns) and, for 16-bit storage, soa16.
+// layout: soa (f32 columns) and, for 16-bit storage, soa16. Sets *extreme
+// (cleared by the caller) when a base colour or SH coefficient is NaN,
+// infinite or too large for a colour sum to stay finite: at least 1e36 (f32,
+// BF16), or at least 65504 (FP16, where it may round to infinity). Otherwise
+// |colour| <= 1e36 * (1 + 15 * 1.1) stays below FLT_MAX at every direction.
 hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
-                       float *soa, uint16_t *soa16);
+                       float *soa, uint16_t *soa16, uint32_t *extreme);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
 // Words of the per-block digit-count table for nseg segments of up to n_cap
@@ -129,11 +133,14 @@ hipError_t launch_ranges(hipStream_t s, const uint32_t *skey, uint32_t key_mask,
 // also sets each unfinished tile's bit in openm (row ty of view v in the words
 // from tile_base + ty * ceil(tiles_x / 32), cleared by the first emission) and
 // counts the view's unfinished tiles in vopen[v].
-// batch = records staged through LDS per step (64, 128 or 256).
+// batch = records staged through LDS per step (64, 128 or 256). exact: the
+// scene may give a splat an infinite colour (launch_prep's flag), which the
+// reference's 4-lane groups turn into NaN beside the lanes that blend; such
+// a scene runs a blend that reproduces them (RGB modes; BM_DIFF ignores it).
 hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                         uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint4 *rec,
                         uint32_t *counters, float *pix_T, uint32_t *pix_last, uint32_t *done, float *pstate,
-                        uint32_t *openm, uint32_t *vopen, int batch);
+                        uint32_t *openm, uint32_t *vopen, int batch, bool exact);
 
 // ---- diff.hip: adjoint of the blend and of the projection ----
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission

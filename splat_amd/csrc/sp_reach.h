@@ -90,6 +90,18 @@ SP_HD inline bool quadrant_reaches(float sx, float sy, float na, float nb2, floa
     return !(pm < thr - power_margin(na, nb2, nc, dxa, dxb, dya, dyb));
 }
 
+// Whether the pixel box [bx0, bx1) x [by0, by1) overlaps quadrant q of the
+// tile at (ox, oy), with no bound on the falloff: the reference adds
+// colour * 0 to the masked lanes of the 4-lane groups that such a record
+// touches, which only an infinite colour makes visible (inf * 0 = NaN).
+SP_HD inline bool quadrant_overlaps(uint32_t bx0, uint32_t by0, uint32_t bx1, uint32_t by1, uint32_t ox, uint32_t oy,
+                                    uint32_t q) {
+    const uint32_t qx0 = ox + 8u * (q & 1u), qy0 = oy + 8u * (q >> 1);
+    const uint32_t xa = bx0 > qx0 ? bx0 : qx0, xb = bx1 < qx0 + 8u ? bx1 : qx0 + 8u;
+    const uint32_t ya = by0 > qy0 ? by0 : qy0, yb = by1 < qy0 + 8u ? by1 : qy0 + 8u;
+    return xa < xb && ya < yb;
+}
+
 // Reach bits of a record over the tile's four quadrants (bit q as above).
 // Record words: w0 = x, y, -a/2, -b; w1 = -c/2, opacity, r, g; w2 = b, box lo
 // (x | y << 16), box hi, threshold.

@@ -379,3 +379,29 @@ def test_needles_match_cpu_bytes(w, h, n, oracle):
         assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
         assert np.array_equal(img.pixels, ref), f"{np.count_nonzero(img.pixels != ref)} bytes differ"
     r.close()
+
+
+def test_infinite_colours_match_cpu_bytes(oracle):
+    """Splats whose colour is +inf (an overflowing SH sum in a loaded file). The
+    cpu backend blends 4-lane groups: every lane of a group with an active lane
+    adds colour * (alpha * T), alpha zeroed where it does not blend, so inf * 0
+    makes NaN channels (byte 0) beside the lanes that blend (src/render/cpu.rs:673-690).
+    The bytes must equal the cpu restatement's, with and without the split."""
+    s = S.random_scene(6000, seed=31, sh_degree=0)
+    rng = np.random.default_rng(31)
+    col = s.colors.copy()
+    hit = rng.random(col.shape) < 0.05
+    col[hit] = np.inf
+    s = S.Scene(s.positions, s.scales, s.rotations, s.opacities, col).finalize()
+    assert np.isinf(s.colors).any()
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    for vi, v in enumerate(_views(s, 2)):
+        ref, rst = cpu.render(v, 640, 360, BG)
+        for shift in (3, 0):
+            case(f"infinite colours view {vi} split shift {shift}")
+            r.set_tuning(blend_batch=64, split_shift=shift, split_kmin=64)
+            img, st = r.render(v, 640, 360, BG)
+            assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
+            assert np.array_equal(img.pixels, ref), f"{np.count_nonzero(img.pixels != ref)} bytes differ"
+    r.close()
