@@ -90,6 +90,9 @@ def main() -> None:
     ap.add_argument("--no-io", action="store_true", help="skip the scene-file leg")
     ap.add_argument("--no-orbit", action="store_true", help="skip the reference-format orbit table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (profiling runs)")
+    ap.add_argument("--tuning", default=None,
+                    help="BATCH,SHIFT,KMIN: pin the blend batch, split shift and split minimum instead of autotuning "
+                         "(profiling runs, where the profiler's own overhead would steer the tuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -131,7 +134,14 @@ def main() -> None:
     out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
     bg = (0.0, 0.0, 0.0)
 
-    if args.no_autotune:
+    pinned = None
+    if args.tuning:
+        b, sh, km = (int(x) for x in args.tuning.split(","))
+        pinned = {"blend_batch": b, "split_shift": sh, "split_kmin": km}
+    if pinned:
+        r.set_tuning(**pinned)
+        tuned = r.tuning()
+    elif args.no_autotune:
         tuned = r.tuning()
     else:
         log("autotune on one orbit view")
@@ -228,7 +238,7 @@ def main() -> None:
 
     if not args.no_batch:
         try:
-            result["batched"] = bench_batched(S, torch, scene, args.batch_views, not args.no_autotune)
+            result["batched"] = bench_batched(S, torch, scene, args.batch_views, not args.no_autotune, pinned)
         except Exception as e:  # reported, not fatal for the headline number
             result["batched"] = {"error": repr(e)}
 
@@ -419,7 +429,7 @@ def bench_long_lists(S, torch, W, H, steps: int) -> dict:
     }
 
 
-def bench_batched(S, torch, scene, nviews: int, autotune: bool = True) -> dict:
+def bench_batched(S, torch, scene, nviews: int, autotune: bool = True, pinned: dict | None = None) -> dict:
     """64 cameras at resolutions from {320x180, 640x360, 1280x720, 1920x1080}, random
     yaw/pitch/zoom: one sp_render_batch call vs one sp_render_device per view."""
     views, sizes = S.random_views(scene, nviews, seed=3)
@@ -427,8 +437,12 @@ def bench_batched(S, torch, scene, nviews: int, autotune: bool = True) -> dict:
     outs = [torch.empty(w * h * 3, dtype=torch.uint8, device="cuda") for (w, h) in sizes]
     ptrs = [o.data_ptr() for o in outs]
     bg = (0.0, 0.0, 0.0)
-    log(f"batched: {nviews} views" + (", autotune" if autotune else ""))
-    tuned = r.autotune(views, sizes, bg, reps=3) if autotune else r.tuning()
+    log(f"batched: {nviews} views" + (", autotune" if autotune and not pinned else ""))
+    if pinned:
+        r.set_tuning(**pinned)
+        tuned = r.tuning()
+    else:
+        tuned = r.autotune(views, sizes, bg, reps=3) if autotune else r.tuning()
     r.render_batch(views, sizes, bg, ptrs, on_device=True)  # warm (captures)
     torch.cuda.synchronize()
     # Both forms: median of 5 repetitions (the sequential loop has host work
