@@ -815,18 +815,18 @@ int render_one(sp_handle *h, const sp_view *view, uint32_t width, uint32_t heigh
 
 // Upload the scene in the handle's storage format. The caller's arrays go to
 // the device as they are, one copy each, and a kernel builds the
-// structure-of-arrays layout (launch_prep); k is computed on host threads
+// blocked column layout (launch_prep, scene_at); k is computed on host threads
 // meanwhile, with the host's logf, as Prep::new does (src/render/cpu.rs:384-388).
 int upload_scene(sp_handle *h, const sp_scene *scene) {
     const uint32_t n = h->n, rest = h->rest;
     const uint64_t S = h->stride;
     const bool f32 = h->opt.storage == SP_STORE_F32;
-    h->soa_bytes = (f32 ? (uint64_t)sp::A_SH + 3ull * rest : 5ull) * S * 4;
+    h->soa_bytes = (f32 ? (uint64_t)sp::A_SH + 3ull * rest : (uint64_t)sp::L_COLS) * S * 4;
     h->soa16_bytes = f32 ? 0 : ((uint64_t)sp::H_SH + 3ull * rest) * S * 2;
     int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
     if (!rc && !f32) rc = dalloc(h, (void **)&h->d_soa16, h->soa16_bytes);
     if (rc) return rc;
-    // Columns are padded to S: the padding reads as zero.
+    // The last block is padded to S splats: the padding reads as zero.
     HIP_TRY(hipMemsetAsync(h->d_soa, 0, h->soa_bytes, h->stream));
     if (!f32) HIP_TRY(hipMemsetAsync(h->d_soa16, 0, h->soa16_bytes, h->stream));
     if (n == 0) {
@@ -880,7 +880,7 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     if (!rc && hipMemcpyAsync(dk, kv.data(), 4ull * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = SP_E_DEVICE;
     hipError_t e = rc ? hipErrorUnknown
-                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, S, h->d_soa, h->d_soa16, d_flag);
+                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, h->d_soa, h->d_soa16, d_flag);
     uint32_t flag = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

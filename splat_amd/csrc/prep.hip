@@ -1,9 +1,9 @@
 // Scene preparation on the device (the upload half of Prep::new,
 // src/render/cpu.rs:349-393): the caller's arrays-of-structures, copied to the
-// device as they are, become the renderer's structure-of-arrays layout in one
-// kernel, one thread per splat. Each output column is written by consecutive
-// threads (coalesced); each splat's own fields are read from its contiguous
-// record.
+// device as they are, become the renderer's blocked column layout (scene_at:
+// blocks of 64 splats, field-major inside a block) in one kernel, one thread
+// per splat. Each output column of a block is written by consecutive threads
+// (coalesced); each splat's own fields are read from its contiguous record.
 //
 // f32 storage: position x3, cov3d x6, opacity, k, base colour x3, SH rest.
 // 16-bit storage: f32 position x3, opacity, k; scale x3, quaternion x4, base
@@ -27,27 +27,28 @@ __device__ __forceinline__ void flag_extreme(bool bad, uint32_t *flag) {
 __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos, const float *__restrict__ cov,
                                                   const float *__restrict__ opac, const float *__restrict__ kk,
                                                   const float *__restrict__ col, const float *__restrict__ sh,
-                                                  uint32_t n, uint32_t rest, uint64_t S, float *__restrict__ soa,
+                                                  uint32_t n, uint32_t rest, float *__restrict__ soa,
                                                   uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     bool bad = false;
+    const uint32_t NC = A_SH + 3 * rest;
     if (i < n) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) soa[(A_POS + c) * S + i] = pos[3ull * i + c];
+        for (int c = 0; c < 3; ++c) soa[scene_at(NC, A_POS + c, i)] = pos[3ull * i + c];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) soa[(A_COV + c) * S + i] = cov[6ull * i + c];
-        soa[A_OPAC * S + i] = opac[i];
-        soa[A_K * S + i] = kk[i];
+        for (int c = 0; c < 6; ++c) soa[scene_at(NC, A_COV + c, i)] = cov[6ull * i + c];
+        soa[scene_at(NC, A_OPAC, i)] = opac[i];
+        soa[scene_at(NC, A_K, i)] = kk[i];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < 1e36f);
-            soa[(A_COL + c) * S + i] = v;
+            soa[scene_at(NC, A_COL + c, i)] = v;
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < 1e36f);
-            soa[(A_SH + c) * S + i] = v;
+            soa[scene_at(NC, A_SH + c, i)] = v;
         }
     }
     flag_extreme(bad, extreme);
@@ -62,31 +63,32 @@ template <bool BF16>
 __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, const float *__restrict__ scl,
                                                  const float *__restrict__ quat, const float *__restrict__ opac,
                                                  const float *__restrict__ kk, const float *__restrict__ col,
-                                                 const float *__restrict__ sh, uint32_t n, uint32_t rest, uint64_t S,
+                                                 const float *__restrict__ sh, uint32_t n, uint32_t rest,
                                                  float *__restrict__ soa, uint16_t *__restrict__ soa16,
                                                  uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const float lim = BF16 ? 1e36f : 65504.0f;
     bool bad = false;
+    const uint32_t NH = H_SH + 3 * rest;
     if (i < n) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) soa[(L_POS + c) * S + i] = pos[3ull * i + c];
-        soa[L_OPAC * S + i] = opac[i];
-        soa[L_K * S + i] = kk[i];
+        for (int c = 0; c < 3; ++c) soa[scene_at(L_COLS, L_POS + c, i)] = pos[3ull * i + c];
+        soa[scene_at(L_COLS, L_OPAC, i)] = opac[i];
+        soa[scene_at(L_COLS, L_K, i)] = kk[i];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) soa16[(H_SCALE + c) * S + i] = to16<BF16>(scl[3ull * i + c]);
+        for (int c = 0; c < 3; ++c) soa16[scene_at(NH, H_SCALE + c, i)] = to16<BF16>(scl[3ull * i + c]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) soa16[(H_QUAT + c) * S + i] = to16<BF16>(quat[4ull * i + c]);
+        for (int c = 0; c < 4; ++c) soa16[scene_at(NH, H_QUAT + c, i)] = to16<BF16>(quat[4ull * i + c]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[(H_COL + c) * S + i] = to16<BF16>(v);
+            soa16[scene_at(NH, H_COL + c, i)] = to16<BF16>(v);
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[(H_SH + c) * S + i] = to16<BF16>(v);
+            soa16[scene_at(NH, H_SH + c, i)] = to16<BF16>(v);
         }
     }
     flag_extreme(bad, extreme);
@@ -94,19 +96,19 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
 
 }  // namespace
 
-hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
-                       float *soa, uint16_t *soa16, uint32_t *extreme) {
+hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, float *soa,
+                       uint16_t *soa16, uint32_t *extreme) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + 255) / 256;
     if (storage == SP_STORE_F32)
         hipLaunchKernelGGL(k_prep_f32, dim3(grid), dim3(256), 0, s, in.pos, in.cov, in.opac, in.k, in.col, in.sh, n,
-                           rest, S, soa, extreme);
+                           rest, soa, extreme);
     else if (storage == SP_STORE_F16)
         hipLaunchKernelGGL(k_prep_16<false>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, S, soa, soa16, extreme);
+                           in.col, in.sh, n, rest, soa, soa16, extreme);
     else
         hipLaunchKernelGGL(k_prep_16<true>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, S, soa, soa16, extreme);
+                           in.col, in.sh, n, rest, soa, soa16, extreme);
     return hipGetLastError();
 }
 

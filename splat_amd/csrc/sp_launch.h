@@ -47,14 +47,14 @@ hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *v
 struct PrepInput {
     const float *pos, *cov, *scl, *quat, *opac, *k, *col, *sh;
 };
-// Writes the structure-of-arrays scene (column stride S >= n) in `storage`'s
-// layout: soa (f32 columns) and, for 16-bit storage, soa16. Sets *extreme
+// Writes the scene's blocked columns (scene_at) in `storage`'s layout: soa
+// (f32 columns) and, for 16-bit storage, soa16. Sets *extreme
 // (cleared by the caller) when a base colour or SH coefficient is NaN,
 // infinite or too large for a colour sum to stay finite: at least 1e36 (f32,
 // BF16), or at least 65504 (FP16, where it may round to infinity). Otherwise
 // |colour| <= 1e36 * (1 + 15 * 1.1) stays below FLT_MAX at every direction.
-hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, uint64_t S,
-                       float *soa, uint16_t *soa16, uint32_t *extreme);
+hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, float *soa,
+                       uint16_t *soa16, uint32_t *extreme);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
 // Words of the per-block digit-count table for nseg segments of up to n_cap

@@ -130,7 +130,6 @@ __device__ __forceinline__ float k_of_opacity(float o) {
 template <int DEG, int STORE>
 __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat<DEG> &s) {
     const float *A = sc.soa;
-    const uint64_t S = sc.stride;
     if (STORE == STORE_PARAMS) {
         const DevParams &P = sc.prm;
         const uint64_t i3 = 3ull * i;
@@ -150,32 +149,34 @@ __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat
 #pragma unroll
         for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = P.sh_rest[sb + c];
     } else if (STORE == SP_STORE_F32) {
-        s.opacity = A[(uint64_t)A_OPAC * S + i];
-        s.k = A[(uint64_t)A_K * S + i];
+        constexpr uint32_t NC = A_SH + 3 * rest_of<DEG>();
+        s.opacity = A[scene_at(NC, A_OPAC, i)];
+        s.k = A[scene_at(NC, A_K, i)];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.p[c] = A[(uint64_t)(A_POS + c) * S + i];
+        for (int c = 0; c < 3; ++c) s.p[c] = A[scene_at(NC, A_POS + c, i)];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s.cv[c] = A[(uint64_t)(A_COV + c) * S + i];
+        for (int c = 0; c < 6; ++c) s.cv[c] = A[scene_at(NC, A_COV + c, i)];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.col[c] = A[(uint64_t)(A_COL + c) * S + i];
+        for (int c = 0; c < 3; ++c) s.col[c] = A[scene_at(NC, A_COL + c, i)];
 #pragma unroll
-        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = A[(uint64_t)(A_SH + c) * S + i];
+        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = A[scene_at(NC, A_SH + c, i)];
     } else {
         const uint16_t *B = sc.soa16;
-        s.opacity = A[(uint64_t)L_OPAC * S + i];
-        s.k = A[(uint64_t)L_K * S + i];
+        constexpr uint32_t NH = H_SH + 3 * rest_of<DEG>();
+        s.opacity = A[scene_at(L_COLS, L_OPAC, i)];
+        s.k = A[scene_at(L_COLS, L_K, i)];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.p[c] = A[(uint64_t)(L_POS + c) * S + i];
+        for (int c = 0; c < 3; ++c) s.p[c] = A[scene_at(L_COLS, L_POS + c, i)];
         float scl[3], q[4];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) scl[c] = h2f<STORE>(B[(uint64_t)(H_SCALE + c) * S + i]);
+        for (int c = 0; c < 3; ++c) scl[c] = h2f<STORE>(B[scene_at(NH, H_SCALE + c, i)]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) q[c] = h2f<STORE>(B[(uint64_t)(H_QUAT + c) * S + i]);
+        for (int c = 0; c < 4; ++c) q[c] = h2f<STORE>(B[scene_at(NH, H_QUAT + c, i)]);
         covariance_f32(scl, q, s.cv);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.col[c] = h2f<STORE>(B[(uint64_t)(H_COL + c) * S + i]);
+        for (int c = 0; c < 3; ++c) s.col[c] = h2f<STORE>(B[scene_at(NH, H_COL + c, i)]);
 #pragma unroll
-        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = h2f<STORE>(B[(uint64_t)(H_SH + c) * S + i]);
+        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = h2f<STORE>(B[scene_at(NH, H_SH + c, i)]);
     }
 }
 
