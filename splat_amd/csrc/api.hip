@@ -1571,7 +1571,7 @@ int sp_autotune(sp_handle *h, const sp_view *views, const uint32_t *wh, uint32_t
         if ((rc = consider(t))) return rc;
     }
     if (cur.split_shift > 0)
-        for (uint32_t km : {4096u, 16384u, 65536u}) {
+        for (uint32_t km : {4096u, 16384u, 65536u, 98304u}) {
             if (km == cur.split_kmin) continue;
             sp_tuning t = cur;
             t.split_kmin = km;
