@@ -300,10 +300,10 @@ def host_cpus_visible() -> int:
 
 
 def host_cores() -> int:
-    """Threads of the host-side legs: every CPU this process may run on, up to the
-    host's share per GPU (16 on an MI355X box, also its OMP_NUM_THREADS)."""
-    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
-    return max(1, min(host_cpus_visible(), share))
+    """Threads of the host-side legs (the CPU baseline, the scene readers): every
+    CPU this process may run on, from its CPU affinity (16 on an MI355X box of
+    this pool, whose share of host CPUs per GPU is 16)."""
+    return max(1, host_cpus_visible())
 
 
 def bench_diff(S, torch, args) -> dict:
