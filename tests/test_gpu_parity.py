@@ -405,3 +405,59 @@ def test_infinite_colours_match_cpu_bytes(oracle):
             assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
             assert np.array_equal(img.pixels, ref), f"{np.count_nonzero(img.pixels != ref)} bytes differ"
     r.close()
+
+
+def _degenerate_scene(n, seed):
+    """A random SH-3 scene with half its splats made degenerate, a fourteenth of
+    that half each: NaN, +inf and -inf position components and positions scaled
+    by 1e30; zero, 1e10, NaN and infinite scales; zero and NaN quaternions
+    (finalize makes them the identity); opacity 0, 1 and NaN (finalize makes
+    NaN 0); NaN SH coefficients."""
+    rng = np.random.default_rng(seed)
+    b = S.random_scene(n, seed=seed, sh_degree=3)
+    pos, sc, rot, op = b.positions.copy(), b.scales.copy(), b.rotations.copy(), b.opacities.copy()
+    col, sh = b.colors.copy(), b.sh_rest.copy()
+    g = np.array_split(rng.permutation(n)[: n // 2], 14)
+    pos[g[0], 0] = np.nan
+    pos[g[1], 2] = np.inf
+    pos[g[2], 1] = -np.inf
+    pos[g[3]] *= np.float32(1e30)
+    sc[g[4]] = 0.0
+    sc[g[5]] = np.float32(1e10)
+    sc[g[6], 1] = np.nan
+    sc[g[7], 0] = np.inf
+    rot[g[8]] = 0.0
+    rot[g[9], 3] = np.nan
+    op[g[10]] = 0.0
+    op[g[11]] = 1.0
+    op[g[12]] = np.nan
+    sh[g[13]] = np.nan
+    return S.Scene(pos, sc, rot, op, col, sh_degree=3, sh_rest=sh).finalize()
+
+
+def test_degenerate_inputs_match_cpu(oracle):
+    """Non-finite and extreme splat parameters: the reference culls whatever its
+    NaN comparisons fail (src/render/mod.rs:202-262) and clamps boxes to the
+    frame. Projection validity and fields, key lists, tile ranges and image bytes
+    must equal the cpu restatement's, with and without the split."""
+    s = _degenerate_scene(6000, 43)
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    for vi, v in enumerate(_views(s)):
+        for (w, h) in [(320, 180), (97, 53)]:
+            case(f"degenerate inputs view {vi} {w}x{h}")
+            gvalid, gproj, gbox = r.debug_project(v, w, h)
+            ovalid, oproj, obox = oracle.project(s, v, w, h)
+            assert np.array_equal(gvalid, ovalid)
+            assert np.array_equal(gproj[gvalid].view(np.uint32), oproj[ovalid].view(np.uint32))
+            assert np.array_equal(gbox[gvalid], obox[ovalid])
+            gk, gv, gt = r.debug_pairs(v, w, h)
+            ok, ov, ot = cpu.pairs(v, w, h)
+            assert np.array_equal(gt, ot) and np.array_equal(gk, ok) and np.array_equal(gv, ov)
+            ref, rst = cpu.render(v, w, h, BG)
+            for shift in (3, 0):
+                r.set_tuning(blend_batch=64, split_shift=shift, split_kmin=64)
+                img, st = r.render(v, w, h, BG)
+                assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
+                assert np.array_equal(img.pixels, ref), f"split {shift}: {np.count_nonzero(img.pixels != ref)} bytes differ"
+    r.close()
