@@ -461,3 +461,25 @@ def test_degenerate_inputs_match_cpu(oracle):
                 assert (st.visible, st.overlaps) == (rst.visible, rst.overlaps)
                 assert np.array_equal(img.pixels, ref), f"split {shift}: {np.count_nonzero(img.pixels != ref)} bytes differ"
     r.close()
+
+
+def test_batch_of_degenerate_scene_and_empty_views(oracle):
+    """The degenerate scene through one batch: random views, a view with the
+    scene behind it (only far-flung splats of the 1e30-scaled group in front),
+    a view from inside the scene and a 1x1 frame. Each view's bytes and stats
+    equal the cpu restatement's, and the batch equals the single-view renders."""
+    s = _degenerate_scene(6000, 47)
+    views, sizes = S.random_views(s, 4, seed=11, sizes=((320, 180), (97, 53)))
+    views += [S.view_at((0.0, 0.0, 100.0)), S.view_at((0.0, 0.0, 0.0), 90.0, 0.01), views[0]]
+    sizes += [(160, 90), (200, 120), (1, 1)]
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    case("degenerate scene, batch of 7 views")
+    imgs, stats = r.render_batch(views, sizes, BG)
+    for k, (v, (w, h)) in enumerate(zip(views, sizes)):
+        ref, rst = cpu.render(v, w, h, BG)
+        assert (stats[k].visible, stats[k].overlaps) == (rst.visible, rst.overlaps), f"view {k}"
+        assert np.array_equal(imgs[k], ref), f"view {k}: {np.count_nonzero(imgs[k] != ref)} bytes differ"
+        one, _ = r.render(v, w, h, BG)
+        assert np.array_equal(one.pixels, ref), f"single view {k}"
+    r.close()
