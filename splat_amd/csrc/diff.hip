@@ -13,7 +13,8 @@
 //   1. k_blend_bwd: one workgroup per tile walks the tile's list back to
 //      front from the last position any of its pixels used, recovering each
 //      pixel's transmittance before a contribution as T_i = T_{i+1} / (1 - a_i)
-//      and carrying the colour accumulated behind it, S. Per pixel and record:
+//      and carrying G . S, S being the colour accumulated behind it. Per
+//      pixel and record:
 //        dL/dc     = G * a * T_i
 //        dL/da     = G . (c * T_i - S / (1 - a))
 //        dL/dop    = dL/da * exp(pw),   dL/dpw = dL/da * a   (0 where a is capped)
@@ -24,9 +25,11 @@
 //      wave, then the four waves);
 //      the mean and conic terms are summed as moments of dL/dpw (dx, dy,
 //      dx^2, dx dy, dy^2) and multiplied by the conic once per record. They are
-//      stored per pair at its emission slot. Records whose falloff cannot
-//      reach a quadrant are never visited by its wave (one ballot over the
-//      staged records' reach bits).
+//      stored per pair at its emission slot. Records are staged as the
+//      forward blend stages them (per-column and per-row power terms, NaN
+//      outside the box, reach bits per quadrant), so every mask decision is
+//      the forward's; records whose falloff cannot reach a quadrant are never
+//      visited by its wave (one ballot over the staged records' reach bits).
 //   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
 //      sums each (view, splat) record's pairs. A per-pair stamp tells the
 //      pairs this backward wrote from stale ones, so nothing is cleared.
@@ -40,6 +43,7 @@
 //      opacity sigmoid and the SH colour, and sums the views.
 #include "sp_launch.h"
 #include "sp_splat.h"
+#include "sp_stage.h"
 
 namespace sp {
 namespace {
@@ -64,10 +68,13 @@ __device__ __forceinline__ float halve_pair(float lo, float hi, bool upper) {
     const float s_lo = lo + dpp_f<CTRL, 0xF>(lo), s_hi = hi + dpp_f<CTRL, 0xF>(hi);
     return upper ? s_hi : s_lo;
 }
+// Pair (lo, nothing): the upper lanes' result is left as the sum too. It
+// stands for nothing there, and every later value computed from it stands for
+// nothing as well (partner lanes hold the same subset in each slot), so it
+// only ever reaches lanes that hold no total (totals9_index gives NG2D).
 template <int CTRL>
-__device__ __forceinline__ float halve_last(float lo, bool upper) {  // pair (lo, nothing)
-    const float s_lo = lo + dpp_f<CTRL, 0xF>(lo);
-    return upper ? 0.0f : s_lo;
+__device__ __forceinline__ float halve_last(float lo) {
+    return lo + dpp_f<CTRL, 0xF>(lo);
 }
 __device__ __forceinline__ float wave_totals9(const float (&g)[NG2D], uint32_t lane) {
     static_assert(NG2D == 9, "the halving steps are laid out for nine values");
@@ -75,12 +82,12 @@ __device__ __forceinline__ float wave_totals9(const float (&g)[NG2D], uint32_t l
     float a[5];
 #pragma unroll
     for (int j = 0; j < 4; ++j) a[j] = halve_pair<0x140>(g[j], g[j + 5], b3);
-    a[4] = halve_last<0x140>(g[4], b3);
+    a[4] = halve_last<0x140>(g[4]);
     float b[3];
 #pragma unroll
     for (int k = 0; k < 2; ++k) b[k] = halve_pair<0x141>(a[k], a[k + 3], b2);
-    b[2] = halve_last<0x141>(a[2], b2);
-    const float c0 = halve_pair<0x4E>(b[0], b[2], b1), c1 = halve_last<0x4E>(b[1], b1);
+    b[2] = halve_last<0x141>(a[2]);
+    const float c0 = halve_pair<0x4E>(b[0], b[2], b1), c1 = halve_last<0x4E>(b[1]);
     float d = halve_pair<0xB1>(c0, c1, b0);
     const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(d), __float_as_uint(d), false, false);
     d = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);  // rows 0+1 and 2+3
@@ -97,16 +104,21 @@ __device__ __forceinline__ uint32_t totals9_index(uint32_t lane) {
     return (bs < 3u && as < 5u && vi < (uint32_t)NG2D) ? vi : (uint32_t)NG2D;
 }
 
+template <int BATCH>
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
     const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ pbase,
     const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
     float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ stamp, uint32_t stamp_id,
     uint32_t total_sbs) {
-    __shared__ uint4 s_rec[BWD_BATCH][3];
-    __shared__ uint32_t s_e[BWD_BATCH];
-    __shared__ uint32_t s_qm[BWD_BATCH];  // quadrant_mask of each staged record
-    __shared__ float s_part[4][NG2D][BWD_BATCH];
+    constexpr int P = 4;  // lanes staging each record
+    static_assert(BATCH <= 64 && BATCH * P <= BLEND_THREADS, "one reach ballot per batch");
+    __shared__ StagedRec s_st[BATCH];
+    __shared__ uint8_t s_qb[BATCH][4];  // reach bit of each staged record per quadrant
+    __shared__ float4 s_geo[BATCH];     // y, -a/2, -b, -c/2 of each staged record
+    __shared__ uint32_t s_e[BATCH];
+    __shared__ float s_part[4][NG2D][BATCH];
+    __shared__ uint64_t s_wrote[4];  // the records each wave left a partial sum for
     __shared__ uint32_t s_wmax[4];
     uint32_t gt, vi;
     if (!blend_tile_of(views, nviews, total_sbs, blockIdx.x, &gt, &vi)) return;
@@ -122,9 +134,6 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const bool inside = px < v.w && py < v.h;
     // The value this lane writes after each nine-value wave reduction.
     const uint32_t sum_slot = lane_i < 16u ? totals9_index(lane_i) : (uint32_t)NG2D;
-    const float fx = (float)ox;
-    const float gx = (float)(lx & ~3u);
-    const float lane = (float)(lx & 3u);
     const float pyc = ((float)oy + (float)ly) + 0.5f;
     const uint64_t g = (uint64_t)gt * BLEND_THREADS + threadIdx.x;
     float T = pix_T[g];
@@ -136,7 +145,9 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         G1 = gi[1];
         G2 = gi[2];
     }
-    float S0 = T * v.bg[0], S1 = T * v.bg[1], S2 = T * v.bg[2];
+    // D = G . S, S being the colour composited behind the current record
+    // (T * bg behind the last one); dL/da needs S only through D.
+    float D = T * ((G0 * v.bg[0] + G1 * v.bg[1]) + G2 * v.bg[2]);
     uint32_t m = last;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
@@ -144,81 +155,71 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     __syncthreads();
     const uint32_t lim = max(max(s_wmax[0], s_wmax[1]), max(s_wmax[2], s_wmax[3]));
     const uint32_t start = tile_start[gt];
-    float *part = &s_part[0][0][0];
     for (uint32_t hi = lim; hi > 0;) {
-        const uint32_t lo = hi > (uint32_t)BWD_BATCH ? hi - BWD_BATCH : 0u;
+        const uint32_t lo = hi > (uint32_t)BATCH ? hi - BATCH : 0u;
         const uint32_t nb = hi - lo;
-        if (threadIdx.x < nb) {
-            const uint32_t id = sval[start + lo + threadIdx.x];  // record index
+        // The forward's staging (same terms, same reach bits), plus what the
+        // gradients need: the record's y and conic, and the pair's emission
+        // slot (the record's first slot plus this tile's row-major place in
+        // the record's tile rectangle, as the emission numbered them).
+        const uint32_t rr = threadIdx.x / P, part = threadIdx.x % P;
+        if (rr < nb) {
+            const uint32_t id = sval[start + lo + rr];
             const uint4 *r = rec + 3 * (uint64_t)id;
             const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
-            // The pair's emission slot: the record's first slot plus this
-            // tile's row-major place in the record's tile rectangle, as the
-            // emission numbered them (tile rectangle from the pixel box).
-            const uint32_t tx0 = (r2.y & 0xFFFFu) / TILE, ty0 = (r2.y >> 16) / TILE;
-            const uint32_t rw = ((r2.z & 0xFFFFu) + TILE - 1u) / TILE - tx0;
-            s_e[threadIdx.x] = pbase[id] + (ty - ty0) * rw + (tx - tx0);
-            s_rec[threadIdx.x][0] = r0;
-            s_rec[threadIdx.x][1] = r1;
-            s_rec[threadIdx.x][2] = r2;
-            s_qm[threadIdx.x] = quadrant_mask(r0, r1, r2, ox, oy);
+            stage_part<P, false>(s_st[rr], s_qb[rr], r0, r1, r2, ox, oy, part);
+            if (part == 0) {
+                const uint32_t tx0 = (r2.y & 0xFFFFu) / TILE, ty0 = (r2.y >> 16) / TILE;
+                const uint32_t rw = ((r2.z & 0xFFFFu) + TILE - 1u) / TILE - tx0;
+                s_e[rr] = pbase[id] + (ty - ty0) * rw + (tx - tx0);
+                s_geo[rr] = make_float4(__uint_as_float(r0.y), __uint_as_float(r0.z), __uint_as_float(r0.w),
+                                        __uint_as_float(r1.x));
+            }
         }
-        for (uint32_t z = threadIdx.x; z < 4u * NG2D * BWD_BATCH; z += BLEND_THREADS) part[z] = 0.0f;
         __syncthreads();
-        // Records that can reach this wave's quadrant (one ballot over the
-        // BWD_BATCH = 64 staged records), walked back to front.
-        static_assert(BWD_BATCH == 64, "one reach ballot per batch");
-        unsigned long long reach = __ballot(lane_i < nb && ((s_qm[lane_i] >> w) & 1u));
+        // Records that can reach this wave's quadrant, walked back to front.
+        unsigned long long reach = lane_mask(lane_i < nb && s_qb[min(lane_i, (uint32_t)BATCH - 1u)][w] != 0u);
+        unsigned long long wrote = 0ull;
         while (reach) {
-            const int k = 63 - __builtin_clzll(reach);
+            const uint32_t k = 63u - (uint32_t)__builtin_clzll(reach);
             reach &= ~(1ull << k);
-            const uint32_t pos = lo + (uint32_t)k;
-            const uint4 r2 = s_rec[k][2];
-            const uint32_t bx0 = r2.y & 0xFFFFu, by0 = r2.y >> 16, bx1 = r2.z & 0xFFFFu, by1 = r2.z >> 16;
-            const bool act = pos < last && px >= bx0 && px < bx1 && py >= by0 && py < by1;
-            const uint64_t act_mask = lane_mask(pos < last) & lane_mask(px >= bx0) & lane_mask(px < bx1) &
-                                      lane_mask(py >= by0) & lane_mask(py < by1);
-            if (act_mask == 0ull) continue;
-            const uint4 r0 = s_rec[k][0];
-            const uint4 r1 = s_rec[k][1];
-            const float sx = __uint_as_float(r0.x), sy = __uint_as_float(r0.y);
-            const float na = __uint_as_float(r0.z), nb2 = __uint_as_float(r0.w);
-            const float nc = __uint_as_float(r1.x);
-            // The forward's exact arithmetic, so every mask decision repeats.
-            const float cx0 = (fx + 0.5f) - sx;
-            const float dx = lane + (gx + cx0);
-            const float dy = pyc - sy;
-            const float qx = (dx * dx) * na;
-            const float qy = (nc * dy) * dy;
-            const float bdy = nb2 * dy;
-            const float pw = (qx + qy) + bdy * dx;
-            if ((act_mask & lane_mask(pw >= __uint_as_float(r2.w))) == 0ull) continue;
-            const float op = __uint_as_float(r1.y);
-            const float ex = exp_neg(pw);
-            const float raw = ex * op;
+            // A lane takes part where the forward's pixel was still live at
+            // this position (before its last contribution), the power is
+            // <= 0 (a NaN term marks a pixel outside the box) and alpha
+            // reaches ALPHA_MIN: the forward's decisions, from the forward's
+            // arithmetic on the same staged terms.
+            const uint64_t am = lane_mask(lo + k < last);
+            if (am == 0ull) continue;
+            const StagedRec &sr = s_st[k];
+            const float2 cv = sr.col[lx], rv = sr.row[ly];
+            const float4 c4 = sr.orgb;
+            const float pw = (cv.y + rv.x) + rv.y * cv.x;
+            int n;
+            const float rp = exp_neg_parts(pw, n);
+            const float raw = __builtin_amdgcn_ldexpf(rp * c4.x, n);
             const float alpha = fminf(raw, ALPHA_MAX);
-            const bool con = act && pw <= 0.0f && alpha >= ALPHA_MIN;
-            // Voted here, where every lane is active, so the masks need no merging.
-            const bool any_con = (act_mask & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN)) != 0ull;
-            if (!any_con) continue;  // no lane takes this record: no gradient, T and S unchanged
+            const uint64_t cm = am & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+            if (cm == 0ull) continue;  // no lane takes this record: no gradient, T and D unchanged
+            const bool con = __builtin_amdgcn_inverse_ballot_w64(cm);
             // Branch-free over the wave: a lane without a contribution gets
-            // zero weights (aT, gs), so its gradients are zero and its T and S
-            // stay as they are.
+            // zero weights (aT, gs), so its gradients are zero and its T and D
+            // stay as they are. The gradient arithmetic uses fused
+            // multiply-adds: only the decisions above must repeat the forward.
+            const float ex = __builtin_amdgcn_ldexpf(rp, n);
+            const float dx = cv.x, dy = pyc - s_geo[k].x;
             const float om = 1.0f - alpha;
             // One hardware reciprocal (1 ulp) for both divisions: gradients
             // need no bit-exactness, and T's recovery drifts by ulps only.
             const float rom = __builtin_amdgcn_rcpf(om);
             const float Ti = T * rom;
-            const float c0 = __uint_as_float(r1.z), c1 = __uint_as_float(r1.w), c2 = __uint_as_float(r2.x);
             const float aT = con ? alpha * Ti : 0.0f;
+            const float Gc = fmaf(G2, c4.w, fmaf(G1, c4.z, G0 * c4.y));
             float gv[NG2D];
             gv[6] = G0 * aT;
             gv[7] = G1 * aT;
             gv[8] = G2 * aT;
-            const float ga = Ti * (G0 * c0 + G1 * c1 + G2 * c2) - (G0 * S0 + G1 * S1 + G2 * S2) * rom;
-            S0 += c0 * aT;
-            S1 += c1 * aT;
-            S2 += c2 * aT;
+            const float ga = fmaf(Ti, Gc, -(D * rom));
+            D = fmaf(Gc, aT, D);
             T = con ? Ti : T;
             // Opacity and falloff terms vanish where alpha is capped.
             const float gs = (con && raw < ALPHA_MAX) ? ga : 0.0f;
@@ -234,21 +235,27 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             gv[4] = wdy * dy;
             const float sum = wave_totals9(gv, lane_i);  // wave-uniform: all 64 lanes take part
             if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
+            wrote |= 1ull << k;
         }
+        if (lane_i == 0) s_wrote[w] = wrote;
         __syncthreads();
         if (threadIdx.x < nb) {
-            const uint32_t e = s_e[threadIdx.x];
+            const uint32_t k = threadIdx.x;
+            const uint32_t e = s_e[k];
+            const uint64_t w0 = s_wrote[0], w1 = s_wrote[1], w2 = s_wrote[2], w3 = s_wrote[3];
+            // A wave that left no partial for this record contributes zero.
+            const auto part_of = [&](uint64_t wm, int ww, int c) {
+                return ((wm >> k) & 1ull) ? s_part[ww][c][k] : 0.0f;
+            };
             float sum[NG2D];
 #pragma unroll
             for (int c = 0; c < NG2D; ++c)
-                sum[c] = (s_part[0][c][threadIdx.x] + s_part[1][c][threadIdx.x]) +
-                         (s_part[2][c][threadIdx.x] + s_part[3][c][threadIdx.x]);
+                sum[c] = (part_of(w0, 0, c) + part_of(w1, 1, c)) + (part_of(w2, 2, c) + part_of(w3, 3, c));
             // dL/dmean = gpw (A dx + B dy, B dx + C dy) and dL/dconic =
             // gpw (-dx^2/2, -dx dy, -dy^2/2), from the moment sums
             // (A = -2 na, B = -nb2, C = -2 nc: the conic a, b, c).
-            const float A = -2.0f * __uint_as_float(s_rec[threadIdx.x][0].z);
-            const float B = -__uint_as_float(s_rec[threadIdx.x][0].w);
-            const float C = -2.0f * __uint_as_float(s_rec[threadIdx.x][1].x);
+            const float4 geo = s_geo[k];
+            const float A = -2.0f * geo.y, B = -geo.z, C = -2.0f * geo.w;
             const float U = sum[0], W = sum[1];
             sum[0] = A * U + B * W;
             sum[1] = B * U + C * W;
@@ -620,7 +627,7 @@ hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nvi
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
                             uint64_t gstride, uint32_t *stamp, uint32_t stamp_id) {
     if (total_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blend_bwd, dim3(blend_grid(total_sbs)), dim3(BLEND_THREADS), 0, s, views, nviews,
+    hipLaunchKernelGGL(k_blend_bwd<BWD_BATCH>, dim3(blend_grid(total_sbs)), dim3(BLEND_THREADS), 0, s, views, nviews,
                        total_tiles, tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id,
                        total_sbs);
     return hipGetLastError();

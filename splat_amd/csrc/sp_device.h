@@ -217,7 +217,9 @@ __device__ __forceinline__ float exp_neg(float xin) {
 // the same value once.
 // (A subnormal product is below ALPHA_MIN, where the blend masks the pixel
 // either way.) One v_ldexp_f32 replaces building 2^n and one multiply.
-__device__ __forceinline__ float exp_neg_times(float xin, float op) {
+// exp_neg_parts returns r and n (exp_neg(x) = r * 2^n) from the same
+// operations.
+__device__ __forceinline__ float exp_neg_parts(float xin, int &n_out) {
     const float x = __builtin_amdgcn_fmed3f(xin, -87.0f, 3.402823466e38f);
     const float n = rintf(x * 1.44269504088896340736f);
     const float f = fmaf(-n, 0.6933594f, x) - n * -2.1219444e-4f;
@@ -228,8 +230,13 @@ __device__ __forceinline__ float exp_neg_times(float xin, float op) {
     p = 1.6666665e-1f + p * f;
     p = 0.5f + p * f;
     const float f2 = f * f;
-    const float r = (f + 1.0f) + p * f2;
-    return __builtin_amdgcn_ldexpf(r * op, (int)n);
+    n_out = (int)n;
+    return (f + 1.0f) + p * f2;
+}
+__device__ __forceinline__ float exp_neg_times(float xin, float op) {
+    int n;
+    const float r = exp_neg_parts(xin, n);
+    return __builtin_amdgcn_ldexpf(r * op, n);
 }
 
 // to_u8 (src/render/mod.rs:282-285), one channel.
