@@ -180,69 +180,123 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         // Records that can reach this wave's quadrant, walked back to front.
         unsigned long long reach = lane_mask(lane_i < nb && s_qb[min(lane_i, (uint32_t)BATCH - 1u)][w] != 0u);
         unsigned long long wrote = 0ull;
-        while (reach) {
-            const uint32_t k = 63u - (uint32_t)__builtin_clzll(reach);
-            reach &= ~(1ull << k);
-            // A lane takes part where the forward's pixel was still live at
-            // this position (before its last contribution), the power is
-            // <= 0 (a NaN term marks a pixel outside the box) and alpha
-            // reaches ALPHA_MIN: the forward's decisions, from the forward's
-            // arithmetic on the same staged terms.
-            const uint64_t am = lane_mask(lo + k < last);
-            if (am == 0ull) continue;
-            const StagedRec &sr = s_st[k];
-            const float2 cv = sr.col[lx], rv = sr.row[ly];
-            const float4 c4 = sr.orgb;
-            const float pw = (cv.y + rv.x) + rv.y * cv.x;
-            int n;
-            const float rp = exp_neg_parts(pw, n);
-            const float raw = __builtin_amdgcn_ldexpf(rp * c4.x, n);
-            const float alpha = fminf(raw, ALPHA_MAX);
-            const uint64_t cm = am & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
-            if (cm == 0ull) continue;  // no lane takes this record: no gradient, T and D unchanged
-            const bool con = __builtin_amdgcn_inverse_ballot_w64(cm);
-            // Branch-free over the wave: a lane without a contribution gets
-            // zero weights (aT, gs), so its gradients are zero and its T and D
-            // stay as they are. The gradient arithmetic uses fused
-            // multiply-adds: only the decisions above must repeat the forward.
-            const float ex = __builtin_amdgcn_ldexpf(rp, n);
-            const float dx = cv.x, dy = pyc - s_geo[k].x;
-            const float om = 1.0f - alpha;
-            // One hardware reciprocal (1 ulp) for both divisions: gradients
-            // need no bit-exactness, and T's recovery drifts by ulps only.
-            const float rom = __builtin_amdgcn_rcpf(om);
-            const float Ti = T * rom;
-            const float aT = con ? alpha * Ti : 0.0f;
-            const float Gc = fmaf(G2, c4.w, fmaf(G1, c4.z, G0 * c4.y));
-            float gv[NG2D];
-            gv[6] = G0 * aT;
-            gv[7] = G1 * aT;
-            gv[8] = G2 * aT;
-            const float ga = fmaf(Ti, Gc, -(D * rom));
-            D = fmaf(Gc, aT, D);
-            T = con ? Ti : T;
-            // Opacity and falloff terms vanish where alpha is capped.
-            const float gs = (con && raw < ALPHA_MAX) ? ga : 0.0f;
-            gv[5] = gs * ex;
-            // Moments of dL/dpw over the pixels; the record's conic
-            // coefficients are applied once per record after the sums.
-            const float gpw = gs * alpha;
-            const float u = gpw * dx, wdy = gpw * dy;
-            gv[0] = u;
-            gv[1] = wdy;
-            gv[2] = u * dx;
-            gv[3] = u * dy;
-            gv[4] = wdy * dy;
-            const float sum = wave_totals9(gv, lane_i);  // wave-uniform: all 64 lanes take part
-            if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][k] = sum;
-            wrote |= 1ull << k;
+        if (reach != 0ull) {
+            // Software-pipelined over two register sets, as the forward blend
+            // walks: a visit reads the next set record's terms from LDS before
+            // its own arithmetic, and the sets alternate. The next record is
+            // the highest set bit (s_flbit clamped: an empty mask gives record
+            // 0 of the batch, an in-bounds read whose values go unused).
+            const char *gbase = reinterpret_cast<const char *>(s_st);
+            const char *pcol = gbase + lx * sizeof(float2);
+            const char *prow = gbase + offsetof(StagedRec, row) + ly * sizeof(float2);
+            // Lane-varying registers for the uniform offsets too, so each
+            // read's address is one vector add.
+            uint32_t orgb_off = (uint32_t)offsetof(StagedRec, orgb), geo_off = 0u;
+            asm("" : "+v"(orgb_off), "+v"(geo_off));
+            const char *porgb = gbase + orgb_off;
+            const char *pgeo = reinterpret_cast<const char *>(s_geo) + geo_off;
+            struct Terms {
+                float2 cv, rv;  // the record's (dx, qx) at this column, (qy, -b dy) at this row
+                float4 c4;      // opacity, colour
+                float sy;       // the record's y
+            };
+            const auto load = [&](uint32_t k, Terms &q) {
+                const uint32_t off = k * (uint32_t)sizeof(StagedRec);
+                q.cv = *reinterpret_cast<const float2 *>(pcol + off);
+                q.rv = *reinterpret_cast<const float2 *>(prow + off);
+                q.c4 = *reinterpret_cast<const float4 *>(porgb + off);
+                q.sy = *reinterpret_cast<const float *>(pgeo + k * (uint32_t)sizeof(float4));
+            };
+            const auto last_set = [](uint64_t mk) {
+                uint32_t z;
+                asm("s_flbit_i32_b64 %0, %1" : "=s"(z) : "s"(mk));
+                return 63u - min(z, 63u);
+            };
+            // Visits record kc (terms q) after reading the terms of the next
+            // set record kn into qn. Returns whether a set record is left
+            // (then kn's bit is cleared).
+            const auto visit = [&](const Terms &q, uint32_t kc, Terms &qn, uint32_t &kn) -> bool {
+                kn = last_set(reach);
+                load(kn, qn);
+                // The arithmetic starts from values this empty asm passes on,
+                // after the reads above (its memory clobber keeps them ahead).
+                float cx = q.cv.x, cy = q.cv.y, rx = q.rv.x, ry = q.rv.y;
+                asm volatile("" : "+v"(cx), "+v"(cy), "+v"(rx), "+v"(ry) : : "memory");
+                // A lane takes part where the forward's pixel was still live
+                // at this position (before its last contribution), the power
+                // is <= 0 (a NaN term marks a pixel outside the box) and
+                // alpha reaches ALPHA_MIN: the forward's decisions, from the
+                // forward's arithmetic on the same staged terms.
+                const uint64_t am = lane_mask(lo + kc < last);
+                if (am != 0ull) {
+                    const float pw = (cy + rx) + ry * cx;
+                    int n;
+                    const float rp = exp_neg_parts(pw, n);
+                    const float raw = __builtin_amdgcn_ldexpf(rp * q.c4.x, n);
+                    const float alpha = fminf(raw, ALPHA_MAX);
+                    const uint64_t cm = am & lane_mask(pw <= 0.0f) & lane_mask(alpha >= ALPHA_MIN);
+                    // No lane taking the record: no gradient, T and D unchanged.
+                    if (cm != 0ull) {
+                        const bool con = __builtin_amdgcn_inverse_ballot_w64(cm);
+                        // Branch-free over the wave: a lane without a
+                        // contribution gets zero weights (aT, gs), so its
+                        // gradients are zero and its T and D stay as they are.
+                        // The gradient arithmetic uses fused multiply-adds:
+                        // only the decisions above must repeat the forward.
+                        const float ex = __builtin_amdgcn_ldexpf(rp, n);
+                        const float dx = cx, dy = pyc - q.sy;
+                        const float om = 1.0f - alpha;
+                        // One hardware reciprocal (1 ulp) for both divisions:
+                        // gradients need no bit-exactness, and T's recovery
+                        // drifts by ulps only.
+                        const float rom = __builtin_amdgcn_rcpf(om);
+                        const float Ti = T * rom;
+                        const float aT = con ? alpha * Ti : 0.0f;
+                        const float Gc = fmaf(G2, q.c4.w, fmaf(G1, q.c4.z, G0 * q.c4.y));
+                        float gv[NG2D];
+                        gv[6] = G0 * aT;
+                        gv[7] = G1 * aT;
+                        gv[8] = G2 * aT;
+                        const float ga = fmaf(Ti, Gc, -(D * rom));
+                        D = fmaf(Gc, aT, D);
+                        T = con ? Ti : T;
+                        // Opacity and falloff terms vanish where alpha is capped.
+                        const float gs = (con && raw < ALPHA_MAX) ? ga : 0.0f;
+                        gv[5] = gs * ex;
+                        // Moments of dL/dpw over the pixels; the record's conic
+                        // coefficients are applied once per record after the sums.
+                        const float gpw = gs * alpha;
+                        const float u = gpw * dx, wdy = gpw * dy;
+                        gv[0] = u;
+                        gv[1] = wdy;
+                        gv[2] = u * dx;
+                        gv[3] = u * dy;
+                        gv[4] = wdy * dy;
+                        const float sum = wave_totals9(gv, lane_i);  // wave-uniform: all 64 lanes take part
+                        if (sum_slot < (uint32_t)NG2D) s_part[w][sum_slot][kc] = sum;
+                        wrote |= 1ull << kc;
+                    }
+                }
+                if (reach == 0ull) return false;
+                asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kn));
+                return true;
+            };
+            uint32_t kA = last_set(reach), kB = 0;
+            asm("s_bitset0_b64 %0, %1" : "+s"(reach) : "s"(kA));
+            Terms qA, qB;
+            load(kA, qA);
+            while (visit(qA, kA, qB, kB) && visit(qB, kB, qA, kA)) {
+            }
         }
         if (lane_i == 0) s_wrote[w] = wrote;
         __syncthreads();
-        if (threadIdx.x < nb) {
+        // A record that no pixel of the tile took has a zero gradient here: its
+        // pair is left unwritten and unstamped, and the reduction counts it as
+        // zero.
+        const uint64_t w0 = s_wrote[0], w1 = s_wrote[1], w2 = s_wrote[2], w3 = s_wrote[3];
+        if (threadIdx.x < nb && (((w0 | w1 | w2 | w3) >> threadIdx.x) & 1ull)) {
             const uint32_t k = threadIdx.x;
             const uint32_t e = s_e[k];
-            const uint64_t w0 = s_wrote[0], w1 = s_wrote[1], w2 = s_wrote[2], w3 = s_wrote[3];
             // A wave that left no partial for this record contributes zero.
             const auto part_of = [&](uint64_t wm, int ww, int c) {
                 return ((wm >> k) & 1ull) ? s_part[ww][c][k] : 0.0f;
