@@ -748,18 +748,25 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_owners(const uint32_t *__
     const uint64_t base = (uint64_t)vbase_pairs[v] + boff[blockIdx.x] + pre;
     const uint32_t b = (uint32_t)min(base, lim), e = (uint32_t)min(base + c, lim);
     if (c > 0) pbase[r.id] = (uint32_t)base;
-    // A short run is filled by its own lane; a long one (a splat over many
-    // tiles) by the whole wave, 64 consecutive slots per store.
-    const bool longrun = e - b > 16u;
-    if (!longrun)
-        for (uint32_t q = b; q < e; ++q) powner[q] = r.id;
-    uint64_t m = lane_mask(longrun);  // every lane of the wave is active here
-    while (m) {
-        const int l = __builtin_ctzll(m);
-        m &= m - 1ull;
-        const uint32_t lb = __builtin_amdgcn_readlane(b, l), le = __builtin_amdgcn_readlane(e, l);
-        const uint32_t lid = __builtin_amdgcn_readlane(r.id, l);
-        for (uint32_t q = lb + lane_id(); q < le; q += 64u) powner[q] = lid;
+    // The wave's splats own one contiguous run of slots, from lane 0's first
+    // slot to lane 63's end (lanes past the visible count hold no pairs). The
+    // wave fills it 64 consecutive slots per store: each lane finds its
+    // slot's owner, the highest lane whose run starts at or before the slot,
+    // by a binary search over the lanes' run starts (lanes with no pairs
+    // share the next run's start and lie below its owner). Every lane of the
+    // wave is active here and in the search.
+    const uint32_t wb = (uint32_t)__builtin_amdgcn_readlane((int)b, 0);
+    const uint32_t we = (uint32_t)__builtin_amdgcn_readlane((int)e, 63);
+    for (uint32_t q0 = wb; q0 < we; q0 += 64u) {
+        const uint32_t q = q0 + lane_id();
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+            const uint32_t cand = lo + step;
+            if ((uint32_t)__shfl((int)b, (int)cand) <= q) lo = cand;
+        }
+        const uint32_t owner = (uint32_t)__shfl((int)r.id, (int)lo);
+        if (q < we) powner[q] = owner;
     }
 }
 
