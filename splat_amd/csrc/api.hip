@@ -201,19 +201,18 @@ struct sp_handle {
     // device memory, per-pair owners, per-pixel blend state, gradient scratch.
     bool diff = false;
     sp::DevParams prm{};
-    uint32_t *d_powner = nullptr;  // pair_cap: record index per emission slot (differentiable handles)
-    uint32_t *d_pbase = nullptr;   // view_cap x n: first emission slot of each record (differentiable handles)
+    // Emission slots of the differentiable handles' tile pairs (sp::PairSlots).
+    uint32_t *d_pbase = nullptr;  // view_cap x n: first emission slot of each record
+    uint32_t *d_pslot = nullptr;  // view_cap x (n + 1): first slot of each depth-ordered splat, views' end slots
+    uint32_t *d_prid = nullptr;   // view_cap x n: record of each depth-ordered splat
     float *d_pixT = nullptr;       // pix_cap: final transmittance per pixel (256 per tile)
     uint32_t *d_pixlast = nullptr; // pix_cap: tile-list position after the last contribution
     uint64_t pix_cap = 0;
     float *d_gpair = nullptr;  // NG2D x gpair_cap per-pair gradients
-    uint32_t *d_stamp = nullptr;  // gpair_cap: id of the backward that last wrote the pair's gradients
-    uint32_t stamp_id = 0;
+    uint32_t *d_fresh = nullptr;  // (gpair_cap + 31) / 32 words: the pairs the running backward wrote
     uint64_t gpair_cap = 0;
     float *d_g2d = nullptr;  // g2d_cap x NG2D per-record gradients, record-major
     uint64_t g2d_cap = 0;
-    float *d_part = nullptr;  // segmented-reduction carries
-    uint64_t part_cap = 0;
     uint32_t fwd_views = 0, fwd_tiles = 0, fwd_sbs = 0;  // shape of the last differentiable forward
     uint64_t fwd_pairs = 0;
     uint32_t last_sbs = 0;  // superblocks of the last rendered batch
@@ -318,7 +317,7 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
     if (cap > 0xFFFFFFFFull) return fail(SP_E_CAPACITY, "more than 2^32 gaussian/tile pairs");
     // The budget is checked before anything is freed: a refused growth leaves
     // the handle's buffers (and its ability to render smaller frames) intact.
-    const uint64_t per_pair = (h->diff ? 5ull : 4ull) * 4;
+    const uint64_t per_pair = 4ull * 4;
     if (h->opt.max_device_bytes && h->bytes - h->pair_cap * per_pair + cap * per_pair > h->opt.max_device_bytes)
         return fail(SP_E_NOMEM, "device memory budget exceeded growing the pair buffers to " + std::to_string(cap) +
                                     " pairs");
@@ -326,18 +325,15 @@ int ensure_pairs(sp_handle *h, uint64_t cap) {
         dfree(h, h->d_pkey[b], h->pair_cap * 4);
         dfree(h, h->d_pval[b], h->pair_cap * 4);
     }
-    dfree(h, h->d_powner, h->pair_cap * 4);
     h->pair_cap = 0;
     for (int b = 0; b < 2; ++b) {
         int rc = dalloc(h, (void **)&h->d_pkey[b], cap * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_pval[b], cap * 4);
-        if (!rc && b == 0 && h->diff) rc = dalloc(h, (void **)&h->d_powner, cap * 4);
         if (rc) {
             for (int c = 0; c < 2; ++c) {
                 dfree(h, h->d_pkey[c], cap * 4);
                 dfree(h, h->d_pval[c], cap * 4);
             }
-            dfree(h, h->d_powner, cap * 4);
             return rc;
         }
     }
@@ -363,7 +359,6 @@ int reset_pairs(sp_handle *h) {
         dfree(h, h->d_pkey[b], h->pair_cap * 4);
         dfree(h, h->d_pval[b], h->pair_cap * 4);
     }
-    dfree(h, h->d_powner, h->pair_cap * 4);
     h->pair_cap = 0;
     return ensure_pairs(h, initial_pairs(h));
 }
@@ -378,7 +373,11 @@ int ensure_views(sp_handle *h, uint32_t views) {
     dfree(h, h->d_rec, old * nn * 48);
     dfree(h, h->d_box, old * nn * 8);
     dfree(h, h->d_sbox, old * nn * 8);
-    if (h->diff) dfree(h, h->d_pbase, old * nn * 4);
+    if (h->diff) {
+        dfree(h, h->d_pbase, old * nn * 4);
+        dfree(h, h->d_pslot, old * (nn + 1) * 4);
+        dfree(h, h->d_prid, old * nn * 4);
+    }
     for (int b = 0; b < 2; ++b) {
         dfree(h, h->d_dkey[b], old * nn * 4);
         dfree(h, h->d_dval[b], old * nn * 4);
@@ -400,6 +399,8 @@ int ensure_views(sp_handle *h, uint32_t views) {
     if (!rc) rc = dalloc(h, (void **)&h->d_box, v * nn * 8);
     if (!rc) rc = dalloc(h, (void **)&h->d_sbox, v * nn * 8);
     if (!rc && h->diff) rc = dalloc(h, (void **)&h->d_pbase, v * nn * 4);
+    if (!rc && h->diff) rc = dalloc(h, (void **)&h->d_pslot, v * (nn + 1) * 4);
+    if (!rc && h->diff) rc = dalloc(h, (void **)&h->d_prid, v * nn * 4);
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = dalloc(h, (void **)&h->d_dkey[b], v * nn * 4);
         if (!rc) rc = dalloc(h, (void **)&h->d_dval[b], v * nn * 4);
@@ -503,14 +504,12 @@ int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
     int rc = SP_OK;
     if (pairs > h->gpair_cap) {
         dfree(h, h->d_gpair, h->gpair_cap * sp::NG2D * 4);
-        dfree(h, h->d_stamp, h->gpair_cap * 4);
+        dfree(h, h->d_fresh, (h->gpair_cap + 31) / 32 * 4);
         h->gpair_cap = 0;
         rc = dalloc(h, (void **)&h->d_gpair, pairs * sp::NG2D * 4);
-        if (!rc) rc = dalloc(h, (void **)&h->d_stamp, pairs * 4);
+        if (!rc) rc = dalloc(h, (void **)&h->d_fresh, (pairs + 31) / 32 * 4);
         if (rc) return rc;
         h->gpair_cap = pairs;
-        h->stamp_id = 0;  // stamps restart: cleared below before the next id is used
-        HIP_TRY(sp::launch_fill(h->stream, h->d_stamp, pairs, 0u));
     }
     if (records > h->g2d_cap) {
         dfree(h, h->d_g2d, h->g2d_cap * sp::NG2D * 4);
@@ -518,14 +517,6 @@ int ensure_grad(sp_handle *h, uint64_t pairs, uint64_t records) {
         rc = dalloc(h, (void **)&h->d_g2d, records * sp::NG2D * 4);
         if (rc) return rc;
         h->g2d_cap = records;
-    }
-    const uint64_t parts = sp::seg_blocks(pairs) * 2 * sp::NG2D;
-    if (parts > h->part_cap) {
-        dfree(h, h->d_part, h->part_cap * 4);
-        h->part_cap = 0;
-        rc = dalloc(h, (void **)&h->d_part, parts * 4);
-        if (rc) return rc;
-        h->part_cap = parts;
     }
     return SP_OK;
 }
@@ -570,7 +561,7 @@ int setup(sp_handle *h) {
 // tile lists of the (last) phase end in ping-pong buffer *pair_buf. The first
 // phase is binned (superblock pairs sorted, tile lists written per
 // superblock); the differentiable path also numbers its tile pairs in
-// emission order (powner, pbase) for the adjoint.
+// emission order (sp::PairSlots) for the adjoint.
 // BL_RGB with a split runs the emission, tile sort, ranges and blend twice:
 // the first phase for each view's nearest splats over all tiles, the second
 // for the rest over the tiles the first blend left unfinished.
@@ -585,7 +576,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     uint32_t *vtot = h->d_vstat + 2 * h->view_cap;
     const bool split = mode == BL_RGB && h->split_shift > 0;
     const sp::EmitSplit es = split ? sp::EmitSplit{h->split_kmin, h->split_shift} : sp::EmitSplit{0u, 0u};
-    uint32_t *powner = mode == BL_DIFF ? h->d_powner : nullptr;
+    sp::PairSlots slots{h->d_pbase, h->d_pslot, h->d_prid};
+    const sp::PairSlots *pslots = mode == BL_DIFF ? &slots : nullptr;
     auto mark = [&](int k) { return events ? hipEventRecord(h->ev[k], s) : hipSuccess; };
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
     HIP_TRY(mark(0));
@@ -604,7 +596,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     HIP_TRY(sp::launch_emit(s, sp::EMIT_FIRST, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box, h->d_views,
                             es, split ? h->d_openm : nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all,
                             vtot, view_pairs,
-                            h->d_bsb, vsb, h->d_pkey[0], h->d_pval[0], powner, powner ? h->d_pbase : nullptr,
+                            h->d_bsb, vsb, h->d_pkey[0], h->d_pval[0], pslots,
                             h->pair_cap, h->d_counters));
     HIP_TRY(mark(1));
     int pi = 0;
@@ -639,7 +631,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     if (split) {
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
                                 h->d_views, es, h->d_openm, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
-                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, nullptr, h->pair_cap, h->d_counters));
+                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
     }
     HIP_TRY(mark(4));
     if (split) {
@@ -980,7 +972,6 @@ uint64_t carve_frame(sp_handle *t, uint8_t *base, uint32_t width, uint32_t heigh
         t->d_pkey[b] = c.take<uint32_t>(pairs * 4);
         t->d_pval[b] = c.take<uint32_t>(pairs * 4);
     }
-    t->d_powner = nullptr;
     t->tile_cap = tiles + 1;
     t->d_tile_start = c.take<uint32_t>((tiles + 1) * 4);
     t->sb_cap = sbs + 1;
@@ -1044,14 +1035,14 @@ void sp_destroy(sp_handle *h) {
     (void)hipFree(h->d_bsb);
     (void)hipFree(h->d_rgb);
     (void)hipFree(h->d_rowtot);
-    (void)hipFree(h->d_powner);
     (void)hipFree(h->d_pbase);
+    (void)hipFree(h->d_pslot);
+    (void)hipFree(h->d_prid);
     (void)hipFree(h->d_pixT);
     (void)hipFree(h->d_pixlast);
     (void)hipFree(h->d_gpair);
-    (void)hipFree(h->d_stamp);
+    (void)hipFree(h->d_fresh);
     (void)hipFree(h->d_g2d);
-    (void)hipFree(h->d_part);
     (void)hipFree(h->d_done);
     (void)hipFree(h->d_openm);
     (void)hipFree(h->d_pstate);
@@ -1667,18 +1658,16 @@ int sp_diff_backward(sp_handle *h, const sp_params *params, const float *const *
     for (uint32_t v = 0; v < nv; ++v) h->h_views[v].gimg = grad_images[v];
     hipStream_t s = h->stream;
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nv * sizeof(ViewParams), hipMemcpyHostToDevice, s));
-    // Per-pair gradients are not cleared: each backward stamps the pairs it
-    // writes with a fresh id and the reduction ignores the others.
-    if (++h->stamp_id == 0) {
-        HIP_TRY(sp::launch_fill(s, h->d_stamp, h->gpair_cap, 0u));
-        h->stamp_id = 1;
-    }
+    // Per-pair gradients are not cleared: each backward marks the pairs it
+    // writes in a bit array cleared here, and the reduction counts the others
+    // as zero.
+    HIP_TRY(sp::launch_fill(s, h->d_fresh, (np + 31) / 32, 0u));
     HIP_TRY(sp::launch_fill_f32(s, h->d_g2d, (uint64_t)nv * h->n * sp::NG2D, 0.0f));  // the records in use
     HIP_TRY(sp::launch_blend_bwd(s, h->d_views, nv, h->fwd_tiles, h->fwd_sbs, h->d_tile_start, h->d_pval[h->pair_buf],
                                  h->d_pbase, h->d_rec, h->d_pixT, h->d_pixlast, h->d_gpair, h->gpair_cap,
-                                 h->d_stamp, h->stamp_id));
-    HIP_TRY(sp::launch_seg_reduce(s, h->d_gpair, h->gpair_cap, h->d_stamp, h->stamp_id, h->d_powner, np, h->d_g2d,
-                                  h->d_part));
+                                 h->d_fresh));
+    HIP_TRY(sp::launch_pair_reduce(s, h->d_gpair, h->gpair_cap, h->d_fresh, h->d_pslot, h->d_prid, h->d_vstat,
+                                   (uint32_t)h->n, nv, h->d_g2d));
     HIP_TRY(sp::launch_project_bwd(s, dev_scene(h), h->d_views, nv, h->d_g2d, grads->means,
                                    grads->log_scales, grads->quats, grads->opacity_logits, grads->colors,
                                    grads->sh_rest));

@@ -4,11 +4,13 @@
 // binned pipeline as the plain path (no front-to-back split). Its emission
 // also numbers the tile pairs in emission order, view by view and splat by
 // splat in depth order, so the pairs of one (view, splat) record are
-// contiguous: the record of each slot (powner) and each record's first slot
-// (pbase); a pair's slot e is pbase[record] plus the tile's row-major place
-// in the record's tile rectangle. The blend keeps, per pixel, its final
-// transmittance and the tile-list position after its last contribution. The backward is three passes, none of which uses a
-// floating-point atomic, so gradients are deterministic:
+// contiguous: each record's first slot (pbase), and the first slot and
+// record of each depth-ordered splat (pslot, prid); a pair's slot e is
+// pbase[record] plus the tile's row-major place in the record's tile
+// rectangle. The blend keeps, per pixel, its final transmittance and the
+// tile-list position after its last contribution. The backward is three
+// passes, none of which uses a floating-point atomic, so gradients are
+// deterministic:
 //
 //   1. k_blend_bwd: one workgroup per tile walks the tile's list back to
 //      front from the last position any of its pixels used, recovering each
@@ -30,14 +32,12 @@
 //      outside the box, reach bits per quadrant), so every mask decision is
 //      the forward's; records whose falloff cannot reach a quadrant are never
 //      visited by its wave (one ballot over the staged records' reach bits).
-//   2. k_seg_reduce + k_seg_carry: a segmented reduction over emission order
-//      sums each (view, splat) record's pairs. A per-pair stamp tells the
-//      pairs this backward wrote from stale ones, so nothing is cleared.
-//      Within a 256-pair block, ballots mark segment heads and written pairs;
-//      each head finds its segment's end from the head mask and adds only
-//      the written pairs, in index order. A segment that crosses blocks is
-//      finished by its first block, which adds the partial sums its
-//      successors left.
+//   2. k_pair_reduce: sums each (view, splat) record's pairs. The pairs
+//      this backward wrote are marked in a bit array (cleared per call), so
+//      the per-pair gradients themselves are never cleared. A workgroup takes
+//      256 consecutive depth-ordered splats of a view, whose pairs are one
+//      contiguous run of emission slots; it stages the run through LDS with
+//      coalesced reads, and each thread adds its own splat's slots in order.
 //   3. k_project_bwd: one thread per splat chains the per-view 2D gradients
 //      through the EWA projection, the covariance (scale, rotation), the
 //      opacity sigmoid and the SH colour, and sums the views.
@@ -49,7 +49,9 @@ namespace sp {
 namespace {
 
 constexpr int BWD_BATCH = 64;  // records per LDS step of the adjoint blend
-constexpr int SR_THREADS = 256;
+constexpr int PR_THREADS = 256;          // depth-ordered splats per workgroup of the pair reduction
+constexpr uint32_t PR_SERIAL = 8;        // marked pairs a thread sums alone
+constexpr uint32_t PR_SERIAL_WORDS = 4;  // mark words a thread scans alone
 constexpr int PBWD_THREADS = 128;
 
 // Totals over the 64 lanes of the NG2D = 9 per-lane values g[] at once, by
@@ -103,14 +105,19 @@ __device__ __forceinline__ uint32_t totals9_index(uint32_t lane) {
     const uint32_t vi = (lane & 8u) ? 5u + as : as;   // of g[0..8]
     return (bs < 3u && as < 5u && vi < (uint32_t)NG2D) ? vi : (uint32_t)NG2D;
 }
+// The lane that holds value c's total after wave_totals9 (the inverse of
+// totals9_index over lanes 0-15).
+__device__ __forceinline__ uint32_t totals9_lane(int c) {
+    constexpr uint32_t lanes[NG2D] = {0, 1, 2, 4, 5, 8, 9, 10, 12};
+    return lanes[c];
+}
 
 template <int BATCH>
 __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
     const ViewParams *__restrict__ views, uint32_t nviews, uint32_t total_tiles,
     const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ sval, const uint32_t *__restrict__ pbase,
     const uint4 *__restrict__ rec, const float *__restrict__ pix_T, const uint32_t *__restrict__ pix_last,
-    float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ stamp, uint32_t stamp_id,
-    uint32_t total_sbs) {
+    float *__restrict__ gpair, uint64_t gstride, uint32_t *__restrict__ fresh, uint32_t total_sbs) {
     constexpr int P = 4;  // lanes staging each record
     static_assert(BATCH <= 64 && BATCH * P <= BLEND_THREADS, "one reach ballot per batch");
     __shared__ StagedRec s_st[BATCH];
@@ -291,7 +298,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
         if (lane_i == 0) s_wrote[w] = wrote;
         __syncthreads();
         // A record that no pixel of the tile took has a zero gradient here: its
-        // pair is left unwritten and unstamped, and the reduction counts it as
+        // pair is left unwritten and unmarked, and the reduction counts it as
         // zero.
         const uint64_t w0 = s_wrote[0], w1 = s_wrote[1], w2 = s_wrote[2], w3 = s_wrote[3];
         if (threadIdx.x < nb && (((w0 | w1 | w2 | w3) >> threadIdx.x) & 1ull)) {
@@ -318,111 +325,112 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_blend_bwd(
             sum[4] = -0.5f * sum[4];
 #pragma unroll
             for (int c = 0; c < NG2D; ++c) gpair[c * gstride + e] = sum[c];
-            stamp[e] = stamp_id;  // this call wrote gpair[.., e]
+            atomicOr(&fresh[e >> 5], 1u << (e & 31u));  // this call wrote gpair[.., e]
         }
         __syncthreads();
         hi = lo;
     }
 }
 
-// Segmented sum over emission order, segments = runs of equal owner (record
-// index). part holds, per block, the partial sum of a segment continuing from
-// the previous block (head, slot 0) and of one continuing into the next
-// (tail, slot 1); a block covered by one segment stores its sum as head.
-__global__ __launch_bounds__(SR_THREADS) void k_seg_reduce(const float *__restrict__ gpair, uint64_t gstride,
-                                                           const uint32_t *__restrict__ stamp, uint32_t stamp_id,
-                                                           const uint32_t *__restrict__ powner, uint64_t np,
-                                                           float *__restrict__ g2d,
-                                                           float *__restrict__ part) {
-    constexpr int NW = SR_THREADS / 64;
-    __shared__ float s_v[NG2D][SR_THREADS];
-    __shared__ uint32_t s_o[SR_THREADS];
-    __shared__ uint64_t s_fresh[NW], s_head[NW];
-    const uint64_t base = (uint64_t)blockIdx.x * SR_THREADS;
-    const uint32_t cnt = (uint32_t)min((uint64_t)SR_THREADS, np - base);
-    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
-    // Pairs past a tile's last used position were not written by this
-    // backward: they contribute zero and are never read.
-    bool fresh = false;
-    if (t < cnt) {
-        s_o[t] = powner[base + t];
-        fresh = stamp[base + t] == stamp_id;
-        if (fresh)
-#pragma unroll
-            for (int c = 0; c < NG2D; ++c) s_v[c][t] = gpair[c * gstride + base + t];
-    }
-    __syncthreads();
-    const uint32_t o = t < cnt ? s_o[t] : 0u;
-    const bool head = t >= cnt || t == 0 || s_o[t - 1] != o;  // first element of a segment (or past the end)
-    const uint64_t fm = __ballot(fresh), hm = __ballot(head);
-    if (lane == 0) {
-        s_fresh[w] = fm;
-        s_head[w] = hm;
-    }
-    __syncthreads();
-    if (t >= cnt || !head) return;
-    // Segment [t, u): u = the next head (or cnt).
-    uint32_t u = cnt;
-    for (uint32_t ww = w; ww < NW; ++ww) {
-        uint64_t m = s_head[ww];
-        if (ww == w) m &= (lane == 63) ? 0ull : (~0ull << (lane + 1));
-        if (m) {
-            u = min(cnt, ww * 64 + (uint32_t)__builtin_ctzll(m));
-            break;
-        }
-    }
+// Bits of fresh word wd that lie in the slot run [b, e).
+__device__ __forceinline__ uint32_t run_bits(uint32_t wd, uint32_t b, uint32_t e) {
+    const uint32_t w0 = wd * 32u;
+    const uint32_t lo = b > w0 ? b - w0 : 0u, hi = e > w0 ? min(e - w0, 32u) : 0u;
+    if (lo >= hi) return 0u;
+    const uint32_t upto = hi == 32u ? 0xFFFFFFFFu : (1u << hi) - 1u;
+    return upto & ~((1u << lo) - 1u);
+}
+
+// Sums each visible record's marked pairs, one thread per depth-ordered
+// splat of a view. Few pairs are marked (a tile's backward stops where its
+// pixels' contributions ended, and a pair no pixel took is not written), so a
+// thread reads the marks of its splat's slot run and adds the marked pairs in
+// slot order. A splat whose run spans more than PR_SERIAL_WORDS mark words,
+// or holds more than PR_SERIAL marked pairs, is summed by its whole wave
+// instead: the wave loads 64 mark words at once, and for each two words that
+// hold marks the lanes take one slot each (lanes 0-31 the first word, 32-63
+// the second), adding its pair to a per-lane sum; one nine-value wave
+// reduction then gives the splat's totals. Either way the order of the
+// additions is fixed, so the sums are deterministic.
+__global__ __launch_bounds__(PR_THREADS) void k_pair_reduce(const float *__restrict__ gpair, uint64_t gstride,
+                                                            const uint32_t *__restrict__ fresh,
+                                                            const uint32_t *__restrict__ pslot,
+                                                            const uint32_t *__restrict__ prid,
+                                                            const uint32_t *__restrict__ vis_count, uint32_t n,
+                                                            uint32_t bpv, float *__restrict__ g2d) {
+    const uint32_t v = blockIdx.x / bpv, blk = blockIdx.x % bpv;
+    const uint32_t nvis = min(vis_count[v], n);
+    if (blk * PR_THREADS >= nvis) return;  // block-uniform: every lane of a live wave stays for the wave path
+    const uint32_t j = blk * PR_THREADS + threadIdx.x, lane = lane_id();
+    const bool valid = j < nvis;
+    const uint32_t *ps = pslot + (uint64_t)v * (n + 1u);
+    const uint32_t b = valid ? ps[j] : 0u, e = valid ? ps[j + 1] : 0u;
+    const uint32_t wfirst = b >> 5, wend = (e + 31u) >> 5;  // mark words [wfirst, wend) cover the run
+    const bool wide = wend - wfirst > PR_SERIAL_WORDS;
+    uint32_t nf = 0;
+    if (!wide)
+        for (uint32_t wd = wfirst; wd < wend; ++wd) nf += __popc(fresh[wd] & run_bits(wd, b, e));
     float sum[NG2D];
 #pragma unroll
     for (int c = 0; c < NG2D; ++c) sum[c] = 0.0f;
-    // Add the written pairs of the segment in index order.
-    for (uint32_t ww = w; ww * 64 < u; ++ww) {
-        uint64_t m = s_fresh[ww];
-        if (ww == w) m &= ~0ull << lane;
-        const uint32_t hi = u - ww * 64;
-        if (hi < 64) m &= (1ull << hi) - 1ull;
-        while (m) {
-            const uint32_t j = ww * 64 + (uint32_t)__builtin_ctzll(m);
-            m &= m - 1ull;
+    if (!wide && nf != 0u && nf <= PR_SERIAL) {
+        for (uint32_t wd = wfirst; wd < wend; ++wd) {
+            uint32_t bits = fresh[wd] & run_bits(wd, b, e);
+            while (bits) {
+                const uint32_t sl = wd * 32u + (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
 #pragma unroll
-            for (int c = 0; c < NG2D; ++c) sum[c] += s_v[c][j];
+                for (int c = 0; c < NG2D; ++c) sum[c] += gpair[c * gstride + sl];
+            }
         }
     }
-    const bool before = t == 0 && base > 0 && powner[base - 1] == o;
-    const bool after = u == cnt && base + cnt < np && powner[base + cnt] == o;
-    if (!before && !after) {
+    bool any = nf != 0u;
+    uint64_t big = lane_mask(wide || nf > PR_SERIAL);
+    while (big) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(big);
+        big &= big - 1ull;
+        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)l);
+        const uint32_t le = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)l);
+        const uint32_t lw0 = lb >> 5, lw1 = (le + 31u) >> 5;
+        float acc[NG2D];
 #pragma unroll
-        for (int c = 0; c < NG2D; ++c) g2d[(uint64_t)o * NG2D + c] = sum[c];
-    } else {
-        float *dst = part + ((uint64_t)blockIdx.x * 2 + (before ? 0 : 1)) * NG2D;
+        for (int c = 0; c < NG2D; ++c) acc[c] = 0.0f;
+        bool marked = false;
+        for (uint32_t base = lw0; base < lw1; base += 64u) {
+            const uint32_t wd = base + lane;
+            const uint32_t word = wd < lw1 ? fresh[wd] & run_bits(wd, lb, le) : 0u;
+            uint64_t wm = lane_mask(word != 0u);
+            marked = marked || wm != 0ull;
+            while (wm) {
+                const uint32_t k1 = (uint32_t)__builtin_ctzll(wm);
+                wm &= wm - 1ull;
+                const uint32_t k2 = wm ? (uint32_t)__builtin_ctzll(wm) : k1;
+                const bool second = wm != 0ull;
+                if (second) wm &= wm - 1ull;
+                const uint32_t k = lane < 32u ? k1 : k2;
+                const uint32_t bits1 = (uint32_t)__builtin_amdgcn_readlane((int)word, (int)k1);
+                const uint32_t bits2 = second ? (uint32_t)__builtin_amdgcn_readlane((int)word, (int)k2) : 0u;
+                const uint32_t bits = lane < 32u ? bits1 : bits2;
+                const uint32_t sl = (base + k) * 32u + (lane & 31u);
+                if ((bits >> (lane & 31u)) & 1u)
 #pragma unroll
-        for (int c = 0; c < NG2D; ++c) dst[c] = sum[c];
+                    for (int c = 0; c < NG2D; ++c) acc[c] += gpair[c * gstride + sl];
+            }
+        }
+        if (!marked) continue;  // wave-uniform
+        const float tot = wave_totals9(acc, lane);  // every lane of the wave takes part
+#pragma unroll
+        for (int c = 0; c < NG2D; ++c) {
+            const float t = __uint_as_float(
+                (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(tot), (int)totals9_lane(c)));
+            if (lane == l) sum[c] = t;
+        }
+        if (lane == l) any = true;
     }
-}
-
-// Finishes segments that cross blocks: the block where such a segment starts
-// adds its tail partial and the head partials of the blocks it runs into.
-__global__ __launch_bounds__(256) void k_seg_carry(const uint32_t *__restrict__ powner, uint64_t np,
-                                                   uint32_t nblocks, const float *__restrict__ part,
-                                                   float *__restrict__ g2d) {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= nblocks) return;
-    const uint64_t base = (uint64_t)b * SR_THREADS;
-    const uint64_t end = min(base + SR_THREADS, np);
-    const uint32_t o = powner[end - 1];
-    if (end >= np || powner[end] != o) return;  // the block's last segment ends inside it
-    if (powner[base] == o && base > 0 && powner[base - 1] == o) return;  // started in an earlier block
-    float sum[NG2D];
+    if (!valid || !any) return;  // records without marked pairs keep the caller's zeros
+    float *dst = g2d + (uint64_t)prid[(uint64_t)v * n + j] * NG2D;
 #pragma unroll
-    for (int c = 0; c < NG2D; ++c) sum[c] = part[((uint64_t)b * 2 + 1) * NG2D + c];
-    for (uint32_t b2 = b + 1; b2 < nblocks; ++b2) {
-#pragma unroll
-        for (int c = 0; c < NG2D; ++c) sum[c] += part[((uint64_t)b2 * 2) * NG2D + c];
-        const uint64_t base2 = (uint64_t)b2 * SR_THREADS;
-        const uint64_t end2 = min(base2 + SR_THREADS, np);
-        if (powner[end2 - 1] != o || end2 >= np || powner[end2] != o) break;
-    }
-#pragma unroll
-    for (int c = 0; c < NG2D; ++c) g2d[(uint64_t)o * NG2D + c] = sum[c];
+    for (int c = 0; c < NG2D; ++c) dst[c] = sum[c];
 }
 
 // Real SH basis of degree DEG (the order sh_color evaluates it) with its
@@ -679,26 +687,20 @@ hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nvi
                             uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval,
                             const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
-                            uint64_t gstride, uint32_t *stamp, uint32_t stamp_id) {
+                            uint64_t gstride, uint32_t *fresh) {
     if (total_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(k_blend_bwd<BWD_BATCH>, dim3(blend_grid(total_sbs)), dim3(BLEND_THREADS), 0, s, views, nviews,
-                       total_tiles, tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, stamp, stamp_id,
-                       total_sbs);
+                       total_tiles, tile_start, sval, pbase, rec, pix_T, pix_last, gpair, gstride, fresh, total_sbs);
     return hipGetLastError();
 }
 
-uint64_t seg_blocks(uint64_t np) { return (np + SR_THREADS - 1) / SR_THREADS; }
-
-hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
-                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, float *part) {
-    if (np == 0) return hipSuccess;
-    const uint64_t nb = seg_blocks(np);
-    hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(SR_THREADS), 0, s, gpair, gstride, stamp, stamp_id,
-                       powner, np, g2d, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_seg_carry, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, powner, np, (uint32_t)nb,
-                       part, g2d);
+hipError_t launch_pair_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *fresh,
+                              const uint32_t *pslot, const uint32_t *prid, const uint32_t *vis_count, uint32_t n,
+                              uint32_t nviews, float *g2d) {
+    if (n == 0 || nviews == 0) return hipSuccess;
+    const uint32_t bpv = (n + PR_THREADS - 1) / PR_THREADS;
+    hipLaunchKernelGGL(k_pair_reduce, dim3(bpv * nviews), dim3(PR_THREADS), 0, s, gpair, gstride, fresh, pslot, prid,
+                       vis_count, n, bpv, g2d);
     return hipGetLastError();
 }
 

@@ -96,15 +96,20 @@ uint32_t emit_blocks(uint32_t n);
 // FIRST is binned (bsb / vsb required, shaped like bcnt / vtot): its pairs are
 // (global superblock id | sub-rectangle << SB_SUB_SHIFT, record index)
 // instead, counted in counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts
-// the tile pairs. With powner (FIRST only), also powner[e] = the
-// record of emission slot e (tile pairs numbered view by view, splat by splat
-// in depth order, each splat's rectangle row-major) and pbase[record] = the
-// record's first slot.
+// the tile pairs. With slots (FIRST only), the emission slots of the tile
+// pairs too, numbered view by view, splat by splat in depth order, each
+// splat's rectangle row-major (see PairSlots).
+struct PairSlots {
+    uint32_t *pbase;  // views x n: a record's first slot (records with pairs)
+    uint32_t *pslot;  // views x (n + 1): view v's j-th depth-ordered splat's first slot at v * (n + 1) + j,
+                      // for j below the view's visible count, and the view's end slot after the last
+    uint32_t *prid;   // views x n: view v's j-th depth-ordered splat's record at v * n + j
+};
 hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t n_splats, uint32_t nviews,
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
-                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
+                       uint32_t *pkey, uint32_t *pval, const PairSlots *slots, uint64_t pair_cap,
                        uint32_t *counters);
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
 // entries, trow bin_scan_rows.
@@ -146,20 +151,21 @@ hipError_t launch_blend(hipStream_t s, int mode, const ViewParams *views, uint32
 // Per-pair gradients (NG2D arrays of gstride floats, indexed by emission
 // slot: pbase[record] + the tile's row-major place in the record's tile
 // rectangle) of every tile's list (record indices, sval) up to its last used
-// position, each written pair stamped with stamp_id; pairs after it are left
-// untouched.
+// position; each pair written is marked in the bit array fresh (bit e % 32 of
+// word e / 32, which the caller clears first), and pairs that no pixel took
+// are left untouched.
 hipError_t launch_blend_bwd(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_tiles,
                             uint32_t total_sbs, const uint32_t *tile_start, const uint32_t *sval, const uint32_t *pbase,
                             const uint4 *rec, const float *pix_T, const uint32_t *pix_last, float *gpair,
-                            uint64_t gstride, uint32_t *stamp, uint32_t stamp_id);
-// Blocks of the segmented reduction over np pairs; part holds 2 * NG2D floats per block.
-uint64_t seg_blocks(uint64_t np);
-// g2d[record * NG2D + c] = sum over the record's pairs of gpair where
-// stamp == stamp_id (others count as zero). Record-major, so each record's
-// sums are one contiguous 36-byte write (records come in depth order, i.e.
-// scattered). Records without pairs are untouched; the caller zero-fills g2d.
-hipError_t launch_seg_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *stamp,
-                             uint32_t stamp_id, const uint32_t *powner, uint64_t np, float *g2d, float *part);
+                            uint64_t gstride, uint32_t *fresh);
+// g2d[record * NG2D + c] = sum over the record's pairs of gpair where the
+// fresh bit is set (others count as zero), for every visible record of the
+// views (pslot / prid as the emission wrote them); record-major, so each
+// record's sums are one 36-byte write. Other records are untouched: the
+// caller zero-fills g2d.
+hipError_t launch_pair_reduce(hipStream_t s, const float *gpair, uint64_t gstride, const uint32_t *fresh,
+                              const uint32_t *pslot, const uint32_t *prid, const uint32_t *vis_count, uint32_t n,
+                              uint32_t nviews, float *g2d);
 // Per-gaussian parameter gradients (sc.storage == STORE_PARAMS), summed over views.
 hipError_t launch_project_bwd(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews,
                               const float *g2d, float *g_means, float *g_log_scales,

@@ -15,11 +15,12 @@
 // workgroup per splat block, each carrying a global tile id (view tile base
 // + tile) and a record index (view * n + scene index), ready for the stable
 // tile-id sort.
-// The differentiable path also numbers the tile pairs in emission order: the
-// record of each emission slot (powner) and each record's first slot
-// (pbase), so that per-pair gradients can be written back in emission order,
-// where each splat's pairs are contiguous (a pair's slot is its record's
-// first slot plus the tile's row-major place in the record's rectangle).
+// The differentiable path also numbers the tile pairs in emission order
+// (each record's first slot, and the first slot and record of each
+// depth-ordered splat), so that per-pair gradients can be written back in
+// emission order, where each splat's pairs are contiguous (a pair's slot is
+// its record's first slot plus the tile's row-major place in the record's
+// rectangle).
 //
 // Front-to-back split. Almost every tile saturates (all pixels reach
 // T < 1e-4) within the first few percent of its depth-ordered list, so the
@@ -721,18 +722,20 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ ske
 // Differentiable path: the emission numbering of the first phase's tile
 // pairs (view by view, splat by splat in depth order, each rectangle
 // row-major), one thread per depth-ordered splat: pbase[record] = the
-// record's first slot and powner[e] = the record of slot e. The slots come
-// from the same block offsets (boff) and view bases (vbase_pairs) as the
-// emission's; slots at or past pair_cap are not written (the frame is then
+// record's first slot, pslot[v * (n + 1) + j] = the first slot of view v's
+// j-th depth-ordered splat (for j < its visible count, plus the view's end
+// slot after the last one) and prid[v * n + j] = that splat's record. The
+// slots come from the same block offsets (boff) and view bases (vbase_pairs)
+// as the emission's; they are clamped to pair_cap (the frame is then
 // rendered again with more room).
-__global__ __launch_bounds__(EMIT_THREADS) void k_emit_owners(const uint32_t *__restrict__ dval, uint32_t n_splats,
-                                                        uint32_t bpv, const uint32_t *__restrict__ vis_count,
-                                                        const uint2 *__restrict__ box,
-                                                        const ViewParams *__restrict__ views, EmitSplit split,
-                                                        const uint32_t *__restrict__ boff,
-                                                        const uint32_t *__restrict__ vbase_pairs,
-                                                        uint32_t *__restrict__ powner, uint32_t *__restrict__ pbase,
-                                                        uint64_t pair_cap) {
+__global__ __launch_bounds__(EMIT_THREADS) void k_emit_slots(const uint32_t *__restrict__ dval, uint32_t n_splats,
+                                                       uint32_t bpv, const uint32_t *__restrict__ vis_count,
+                                                       const uint2 *__restrict__ box,
+                                                       const ViewParams *__restrict__ views, EmitSplit split,
+                                                       const uint32_t *__restrict__ boff,
+                                                       const uint32_t *__restrict__ vbase_pairs,
+                                                       uint32_t *__restrict__ pbase, uint32_t *__restrict__ pslot,
+                                                       uint32_t *__restrict__ prid, uint64_t pair_cap) {
     const uint32_t v = blockIdx.x / bpv, blk = blockIdx.x % bpv;
     const uint64_t vbase = (uint64_t)v * n_splats;
     const uint32_t nvis = min(vis_count[v], n_splats);
@@ -746,28 +749,13 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_owners(const uint32_t *__
     // Slots are u32 (list entries are); clamped to the capacity.
     const uint64_t lim = min(pair_cap, (uint64_t)0xFFFFFFFFu);
     const uint64_t base = (uint64_t)vbase_pairs[v] + boff[blockIdx.x] + pre;
-    const uint32_t b = (uint32_t)min(base, lim), e = (uint32_t)min(base + c, lim);
-    if (c > 0) pbase[r.id] = (uint32_t)base;
-    // The wave's splats own one contiguous run of slots, from lane 0's first
-    // slot to lane 63's end (lanes past the visible count hold no pairs). The
-    // wave fills it 64 consecutive slots per store: each lane finds its
-    // slot's owner, the highest lane whose run starts at or before the slot,
-    // by a binary search over the lanes' run starts (lanes with no pairs
-    // share the next run's start and lie below its owner). Every lane of the
-    // wave is active here and in the search.
-    const uint32_t wb = (uint32_t)__builtin_amdgcn_readlane((int)b, 0);
-    const uint32_t we = (uint32_t)__builtin_amdgcn_readlane((int)e, 63);
-    for (uint32_t q0 = wb; q0 < we; q0 += 64u) {
-        const uint32_t q = q0 + lane_id();
-        uint32_t lo = 0;
-#pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1) {
-            const uint32_t cand = lo + step;
-            if ((uint32_t)__shfl((int)b, (int)cand) <= q) lo = cand;
-        }
-        const uint32_t owner = (uint32_t)__shfl((int)r.id, (int)lo);
-        if (q < we) powner[q] = owner;
-    }
+    if (j >= nvis) return;
+    const uint32_t b = (uint32_t)min(base, lim);
+    if (c > 0) pbase[r.id] = b;
+    uint32_t *ps = pslot + (uint64_t)v * (n_splats + 1u);
+    ps[j] = b;
+    if (j == nvis - 1u) ps[nvis] = (uint32_t)min(base + c, lim);
+    prid[vbase + j] = r.id;
 }
 
 template <int PHASE>
@@ -776,8 +764,7 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       EmitSplit split,
                       uint32_t *openm, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *bsb, uint32_t *vsb, uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
-                      uint32_t *powner, uint32_t *pbase, uint64_t pair_cap, uint32_t count_slot,
-                      uint32_t *counters) {
+                      const PairSlots *slots, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
     const uint32_t bpv = emit_blocks(n_splats);
     const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
@@ -803,11 +790,12 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, EXPAND_GRID));
         hipLaunchKernelGGL(k_emit_expand_sb, dim3(grid), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
                            nviews, vis_count, box, views, split, bsb, vsb, pkey, pval, pair_cap, counters);
-        if (powner) {  // the differentiable path also numbers the tile pairs in emission order
+        if (slots) {  // the differentiable path also numbers the tile pairs in emission order
             const hipError_t eb = hipGetLastError();
             if (eb != hipSuccess) return eb;
-            hipLaunchKernelGGL(k_emit_owners, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                               vis_count, box, views, split, bcnt, vtot, powner, pbase, pair_cap);
+            hipLaunchKernelGGL(k_emit_slots, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
+                               vis_count, box, views, split, bcnt, vtot, slots->pbase, slots->pslot, slots->prid,
+                               pair_cap);
         }
     } else if (PHASE == EMIT_SECOND) {
         hipLaunchKernelGGL(k_emit_expand_open, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
@@ -824,15 +812,15 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
                        const uint32_t *vis_count, const uint2 *sbox, uint2 *box, const ViewParams *views,
                        EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
-                       uint32_t *pkey, uint32_t *pval, uint32_t *powner, uint32_t *pbase, uint64_t pair_cap,
+                       uint32_t *pkey, uint32_t *pval, const PairSlots *slots, uint64_t pair_cap,
                        uint32_t *counters) {
     if (phase == EMIT_SECOND)
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt,
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, nullptr,
-                                       nullptr, pair_cap, CNT_PAIRS2, counters);
+                                       pair_cap, CNT_PAIRS2, counters);
     if (!bsb || !vsb) return hipErrorInvalidValue;  // the first phase is always binned
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt, bcnt_all,
-                                  vtot, view_pairs, bsb, vsb, vopen, pkey, pval, powner, pbase, pair_cap, CNT_PAIRS,
+                                  vtot, view_pairs, bsb, vsb, vopen, pkey, pval, slots, pair_cap, CNT_PAIRS,
                                   counters);
 }
 
