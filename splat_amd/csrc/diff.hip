@@ -50,7 +50,7 @@ namespace {
 
 constexpr int BWD_BATCH = 64;  // records per LDS step of the adjoint blend
 constexpr int PR_THREADS = 256;          // depth-ordered splats per workgroup of the pair reduction
-constexpr uint32_t PR_SERIAL = 8;        // marked pairs a thread sums alone
+constexpr uint32_t PR_SERIAL = 4;        // marked pairs a thread sums alone
 constexpr uint32_t PR_SERIAL_WORDS = 4;  // mark words a thread scans alone
 constexpr int PBWD_THREADS = 128;
 
@@ -367,22 +367,44 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_reduce(const float *__restr
     const uint32_t b = valid ? ps[j] : 0u, e = valid ? ps[j + 1] : 0u;
     const uint32_t wfirst = b >> 5, wend = (e + 31u) >> 5;  // mark words [wfirst, wend) cover the run
     const bool wide = wend - wfirst > PR_SERIAL_WORDS;
+    static_assert(PR_SERIAL_WORDS == 4, "the slot pick below reads four words");
+    // A narrow run's mark words, read at once.
+    uint32_t wds[PR_SERIAL_WORDS];
     uint32_t nf = 0;
-    if (!wide)
-        for (uint32_t wd = wfirst; wd < wend; ++wd) nf += __popc(fresh[wd] & run_bits(wd, b, e));
+#pragma unroll
+    for (uint32_t i = 0; i < PR_SERIAL_WORDS; ++i) {
+        const uint32_t wd = wfirst + i;
+        wds[i] = (!wide && wd < wend) ? fresh[wd] & run_bits(wd, b, e) : 0u;
+        nf += __popc(wds[i]);
+    }
     float sum[NG2D];
 #pragma unroll
     for (int c = 0; c < NG2D; ++c) sum[c] = 0.0f;
     if (!wide && nf != 0u && nf <= PR_SERIAL) {
-        for (uint32_t wd = wfirst; wd < wend; ++wd) {
-            uint32_t bits = fresh[wd] & run_bits(wd, b, e);
-            while (bits) {
-                const uint32_t sl = wd * 32u + (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
+        // The marked slots in order, then all their reads at once, then the
+        // sums in slot order (an absent slot adds zero, which changes nothing).
+        uint32_t sl[PR_SERIAL];
 #pragma unroll
-                for (int c = 0; c < NG2D; ++c) sum[c] += gpair[c * gstride + sl];
-            }
+        for (uint32_t k = 0; k < PR_SERIAL; ++k) {
+            uint32_t i = 0;
+#pragma unroll
+            for (uint32_t q = PR_SERIAL_WORDS; q-- > 0;)
+                if (wds[q]) i = q;
+            const uint32_t w = wds[0] ? wds[0] : wds[1] ? wds[1] : wds[2] ? wds[2] : wds[3];
+            sl[k] = (wfirst + i) * 32u + (w ? (uint32_t)__builtin_ctz(w) : 0u);
+#pragma unroll
+            for (uint32_t q = 0; q < PR_SERIAL_WORDS; ++q)
+                if (q == i) wds[q] &= wds[q] - 1u;
         }
+        float val[PR_SERIAL][NG2D];
+#pragma unroll
+        for (uint32_t k = 0; k < PR_SERIAL; ++k)
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c) val[k][c] = k < nf ? gpair[c * gstride + sl[k]] : 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < PR_SERIAL; ++k)
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c) sum[c] += val[k][c];
     }
     bool any = nf != 0u;
     uint64_t big = lane_mask(wide || nf > PR_SERIAL);
@@ -515,14 +537,23 @@ __global__ __launch_bounds__(PBWD_THREADS) void k_project_bwd(DevScene sc, const
     for (int c = 0; c < 3 * R; ++c) gsh[c] = 0.0f;
     float Gs[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};  // sum of G + G^T
     float gop = 0.0f;
+    // Each view's 2D gradients are read during the previous view's
+    // arithmetic, so the loop does not wait for a load at every view.
+    float gn[NG2D];
+#pragma unroll
+    for (int c = 0; c < NG2D; ++c) gn[c] = nviews ? g2d[(uint64_t)i * NG2D + c] : 0.0f;
     for (uint32_t vv = 0; vv < nviews; ++vv) {
-        const uint64_t rid = (uint64_t)vv * sc.n + i;
         float g[NG2D];
         bool any = false;
 #pragma unroll
         for (int c = 0; c < NG2D; ++c) {
-            g[c] = g2d[rid * NG2D + c];
+            g[c] = gn[c];
             any = any || g[c] != 0.0f;
+        }
+        if (vv + 1 < nviews) {
+            const uint64_t nrid = (uint64_t)(vv + 1) * sc.n + i;
+#pragma unroll
+            for (int c = 0; c < NG2D; ++c) gn[c] = g2d[nrid * NG2D + c];
         }
         if (!any) continue;
         const ViewCtx &v = views[vv].ctx;
