@@ -233,6 +233,7 @@ typedef struct sp_frame_info {
     uint64_t pairs_staged;   /* pair records the blend gathered before its tiles saturated */
     uint64_t pair_capacity;  /* current pair buffer capacity */
     uint64_t device_bytes;   /* device memory held by the handle */
+    uint64_t depth_key_bits; /* key width of the last depth sort: 27 (depths bounded near the views' near planes) or 32 */
 } sp_frame_info;
 
 int sp_last_frame_info(const sp_handle *h, sp_frame_info *info);

@@ -120,6 +120,7 @@ class SpFrameInfo(C.Structure):
         ("pairs_staged", C.c_uint64),
         ("pair_capacity", C.c_uint64),
         ("device_bytes", C.c_uint64),
+        ("depth_key_bits", C.c_uint64),
     ]
 
 

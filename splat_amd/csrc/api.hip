@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -93,6 +94,15 @@ ViewCtx make_ctx(const sp_view &v, uint32_t w, uint32_t h, const float bg[3]) {
     return c;
 }
 
+// The depth-sort key base of a view (ViewCtx::key_base): the near plane's
+// bits when every visible depth of the scene in this view lies below the
+// near plane's bits plus 2^DEPTH_KEY_BITS, else 0 (full 32-bit keys). A
+// visible splat has near_z < depth <= |forward| * |p - eye| (up to the f32
+// rounding of the projection, covered by a relative margin of 2^-16), and
+// |p - eye| <= |c - eye| + r over the bounding sphere (c, r) of the scene's
+// finite positions.
+uint32_t depth_key_base(const sp_handle *h, const ViewCtx &c);
+
 // What a batch's blend stage produces.
 constexpr int BL_NONE = 0;  // stop after the tile ranges
 constexpr int BL_RGB = 1;   // RGB8 images (Renderer::render)
@@ -109,9 +119,11 @@ double now_ms() {
 struct GraphKey {
     uint32_t nviews = 0, total_tiles = 0, tile_bits = 0, with_blend = 0;
     uint32_t split_shift = 0, split_kmin = 0, blend_batch = 0, total_sbs = 0;
+    uint32_t depth_bits = 0;
     uint64_t generation = 0;
     bool operator==(const GraphKey &o) const {
         return nviews == o.nviews && total_tiles == o.total_tiles && tile_bits == o.tile_bits &&
+               depth_bits == o.depth_bits &&
                total_sbs == o.total_sbs &&
                with_blend == o.with_blend && split_shift == o.split_shift && split_kmin == o.split_kmin &&
                blend_batch == o.blend_batch && generation == o.generation;
@@ -147,6 +159,13 @@ struct sp_handle {
     // frames blend through the instance that reproduces the reference's
     // NaN channels beside such splats.
     bool extreme_colour = false;
+    // Bounding sphere of the scene's finite positions (found at upload; not
+    // for differentiable handles, whose positions come with each call): it
+    // bounds each view's depths, so the depth sort can use DEPTH_KEY_BITS-bit
+    // keys. depth_full is set for good if a frame ever found a key beyond it.
+    bool bound_ok = false, depth_full = false;
+    double bound_c[3] = {0.0, 0.0, 0.0}, bound_r = 0.0;
+    int depth_bits = 32;  // key width of the last batch's depth sort
     // Per-view arrays, view_cap views of n entries each.
     uint32_t view_cap = 0;
     uint4 *d_rec = nullptr;
@@ -225,6 +244,23 @@ struct sp_handle {
 };
 
 namespace {
+
+uint32_t depth_key_base(const sp_handle *h, const ViewCtx &c) {
+    if (!h->bound_ok || h->depth_full || !(c.near_z > 0.0f) || !std::isfinite(c.near_z)) return 0u;
+    double d2 = 0.0, f2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        d2 += ((double)c.eye[k] - h->bound_c[k]) * ((double)c.eye[k] - h->bound_c[k]);
+        f2 += (double)c.rot[2][k] * (double)c.rot[2][k];
+    }
+    const double far = std::sqrt(f2) * (std::sqrt(d2) + h->bound_r) * (1.0 + 1.0 / 65536.0);
+    if (!(far < 3.0e38)) return 0u;
+    uint32_t near_bits, far_bits;
+    const float nf = c.near_z, ff = std::nextafter((float)far, INFINITY);
+    std::memcpy(&near_bits, &nf, 4);
+    std::memcpy(&far_bits, &ff, 4);
+    if (far_bits <= near_bits) return near_bits;  // every splat lies at or before the near plane
+    return far_bits - near_bits < (1u << sp::DEPTH_KEY_BITS) ? near_bits : 0u;
+}
 
 int dalloc(sp_handle *h, void **p, uint64_t bytes) {
     *p = nullptr;
@@ -585,11 +621,11 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     // depth sort's sparse first pass compacts them (scene-index order within a
     // digit, so the sort stays stable) and writes each view's visible count.
     HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_sbox, h->d_dkey[0], nullptr, nullptr,
-                               nullptr));
+                               nullptr, h->d_counters));
     if (h->n == 0) HIP_TRY(sp::launch_fill(s, vis, nviews, 0u));  // no sort runs to count them
     int di = 0;
-    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, 32, h->d_rowtot, h->d_table, &di,
-                                  true));
+    HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, h->depth_bits, h->d_rowtot,
+                                  h->d_table, &di, true));
     h->depth_buf = di;
     uint32_t *vsb = h->d_vstat + 3 * h->view_cap;
     uint32_t *vopen = h->d_vstat + 4 * h->view_cap;
@@ -686,6 +722,19 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     }
     if (total >= 0xFFFFFFFFull) return fail(SP_E_INVALID, "too many tiles in one batch");
     const uint32_t total_tiles = (uint32_t)total;
+    // Depth keys: DEPTH_KEY_BITS wide when every view's depths are bounded
+    // close enough to its near plane, else the full 32 bits for the batch.
+    const auto plan_depth = [&] {
+        bool narrow = true;
+        for (uint32_t v = 0; v < nviews; ++v) {
+            h->h_views[v].ctx.key_base = depth_key_base(h, h->h_views[v].ctx);
+            narrow = narrow && h->h_views[v].ctx.key_base != 0u;
+        }
+        if (!narrow)
+            for (uint32_t v = 0; v < nviews; ++v) h->h_views[v].ctx.key_base = 0u;
+        h->depth_bits = narrow ? sp::DEPTH_KEY_BITS : 32;
+    };
+    plan_depth();
     rc = ensure_tiles(h, total_tiles);
     if (!rc && mode == BL_DIFF) rc = ensure_pix(h, total_tiles);
     if (!rc && total_sb >= (1ull << sp::SB_SUB_SHIFT))
@@ -707,6 +756,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
             key.split_kmin = h->split_kmin;
             key.blend_batch = (uint32_t)h->blend_batch;
             key.total_sbs = (uint32_t)total_sb;
+            key.depth_bits = (uint32_t)h->depth_bits;
             key.generation = h->generation;
             bool hit = false;
             GraphEntry &ge = graph_slot(h, key, &hit);
@@ -736,6 +786,14 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
         }
         HIP_TRY(hipStreamSynchronize(h->stream));
         h->pair_buf = pb;
+        if (h->h_counters[sp::CNT_DEPTH_RANGE]) {
+            // A depth key did not fit the planned width (the bound failed):
+            // this handle sorts full 32-bit depth keys from now on.
+            h->depth_full = true;
+            HIP_TRY(sp::launch_fill(h->stream, h->d_counters + sp::CNT_DEPTH_RANGE, 1, 0u));
+            plan_depth();
+            continue;
+        }
         // Pair counts saturate at 2^32 - 1 (the scans never wrap): more pairs
         // than a u32 index addresses cannot be rendered (the reference's CPU
         // backend stops there too, src/render/cpu.rs:158-160).
@@ -793,6 +851,7 @@ int run_views(sp_handle *h, uint32_t nviews, const sp_view *views, const uint32_
     h->last.pairs_staged = blend ? h->h_counters[sp::CNT_STAGED] : 0;
     h->last.pair_capacity = h->pair_cap;
     h->last.device_bytes = h->bytes;
+    h->last.depth_key_bits = (uint64_t)h->depth_bits;
     return SP_OK;
 }
 
@@ -825,15 +884,27 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
         HIP_TRY(hipStreamSynchronize(h->stream));
         return SP_OK;
     }
-    // k on host threads while the copies run.
+    // k and the bounding box of the finite positions on host threads while
+    // the copies run.
     std::vector<float> kv(n);
     const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const uint32_t nt = std::max(1u, std::min(hw, n / 65536u));
+    std::vector<std::array<double, 6>> boxes(nt);
     std::vector<std::thread> pool;
     for (uint32_t t = 0; t < nt; ++t)
         pool.emplace_back([&, t] {
             const uint64_t b = (uint64_t)n * t / nt, e = (uint64_t)n * (t + 1) / nt;
             for (uint64_t i = b; i < e; ++i) kv[i] = k_of(scene->opacities[i]);
+            std::array<double, 6> &bb = boxes[t];
+            bb = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            for (uint64_t i = b; i < e; ++i) {
+                const float *q = scene->positions + 3 * i;
+                if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2])) continue;
+                for (int c = 0; c < 3; ++c) {
+                    bb[c] = std::min(bb[c], (double)q[c]);
+                    bb[3 + c] = std::max(bb[3 + c], (double)q[c]);
+                }
+            }
         });
     // Staging: the caller's arrays back to back.
     const uint64_t geo = f32 ? 6ull : 7ull;  // cov3d, or scale + quaternion
@@ -869,6 +940,21 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     uint32_t *d_flag = reinterpret_cast<uint32_t *>(p);
     if (!rc && hipMemsetAsync(d_flag, 0, 4, h->stream) != hipSuccess) rc = SP_E_DEVICE;
     for (auto &th : pool) th.join();
+    // Splats with a non-finite position are culled in every view, so the box
+    // of the others bounds every visible depth.
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const auto &bb : boxes)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], bb[c]);
+            hi[c] = std::max(hi[c], bb[3 + c]);
+        }
+    double r2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        h->bound_c[c] = 0.5 * (lo[c] + hi[c]);
+        r2 += 0.25 * (hi[c] - lo[c]) * (hi[c] - lo[c]);
+    }
+    h->bound_r = std::sqrt(r2);
+    h->bound_ok = lo[0] <= hi[0] && std::isfinite(h->bound_r);
     if (!rc && hipMemcpyAsync(dk, kv.data(), 4ull * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = SP_E_DEVICE;
     hipError_t e = rc ? hipErrorUnknown
@@ -1282,7 +1368,7 @@ int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     if (e == hipSuccess) e = hipMemcpyAsync(h->d_views, h->h_views, sizeof(ViewParams), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
         e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_sbox, h->d_dkey[0], d_dbg, d_bb,
-                               d_valid);
+                               d_valid, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(proj, d_dbg, n * 48, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(bbox, d_bb, n * 8, hipMemcpyDeviceToHost);
@@ -1322,7 +1408,8 @@ int sp_debug_pairs(sp_handle *h, const sp_view *view, uint32_t width, uint32_t h
     std::vector<uint32_t> dk(nv), dv(nv), depth(h->n, 0u);
     HIP_TRY(hipMemcpy(dk.data(), h->d_dkey[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(dv.data(), h->d_dval[h->depth_buf], nv * 4ull, hipMemcpyDeviceToHost));
-    for (uint32_t q = 0; q < nv; ++q) depth[dv[q]] = dk[q];
+    const uint32_t kb = h->h_views[0].ctx.key_base;  // the sort's keys are the depth bits less the view's base
+    for (uint32_t q = 0; q < nv; ++q) depth[dv[q]] = dk[q] + kb;
     for (uint64_t p = 0; p < np; ++p) keys[p] = ((uint64_t)tk[p] << 32) | depth[vals[p]];
     return SP_OK;
 }

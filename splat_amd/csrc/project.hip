@@ -28,7 +28,8 @@ __global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, const 
                                                              uint32_t *__restrict__ full,
                                                              float *__restrict__ dbg,
                                                              uint16_t *__restrict__ dbg_bbox,
-                                                             uint8_t *__restrict__ dbg_valid) {
+                                                             uint8_t *__restrict__ dbg_valid,
+                                                             uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * PROJECT_THREADS + threadIdx.x;
     if (i >= sc.n) return;  // no barriers follow
     const uint64_t N = sc.n;
@@ -44,7 +45,15 @@ __global__ __launch_bounds__(PROJECT_THREADS) void k_project(DevScene sc, const 
                 float depth = 0.0f;
                 const bool vis = project_view<DEG, STORE != SP_STORE_F32>(
                     s, views[v].ctx, rec + 3 * ((uint64_t)v * N + i), &depth, g, gb, sbox + (uint64_t)v * N + i);
-                full[(uint64_t)v * N + i] = vis ? __float_as_uint(depth) : FULL_CULLED;
+                // Visible depths lie above the near plane, so their bits exceed
+                // key_base (the near plane's bits, or 0).
+                const uint32_t kb = views[v].ctx.key_base, key = __float_as_uint(depth) - kb;
+                full[(uint64_t)v * N + i] = vis ? key : FULL_CULLED;
+                // The host's depth bound is conservative; should a key not fit
+                // the planned width anyway, the flag makes it render the frame
+                // again with full keys.
+                if (vis && kb != 0u && (key >> DEPTH_KEY_BITS) != 0u && counters)
+                    atomicOr(&counters[CNT_DEPTH_RANGE], 1u);
                 if (dbg_valid && v == 0) dbg_valid[i] = vis ? 1 : 0;
             }
         }
@@ -58,13 +67,14 @@ struct ProjectOut {
     float *dbg;
     uint16_t *dbg_bbox;
     uint8_t *dbg_valid;
+    uint32_t *counters;
 };
 
 template <int DEG, int STORE>
 void launch_deg(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewParams *views, uint32_t nviews,
                 const ProjectOut &o) {
     hipLaunchKernelGGL((k_project<DEG, STORE>), dim3(grid), dim3(PROJECT_THREADS), 0, s, sc, views, nviews, o.rec,
-                       o.sbox, o.full, o.dbg, o.dbg_bbox, o.dbg_valid);
+                       o.sbox, o.full, o.dbg, o.dbg_bbox, o.dbg_valid, o.counters);
 }
 
 template <int STORE>
@@ -83,10 +93,11 @@ void launch_store(hipStream_t s, uint32_t grid, const DevScene &sc, const ViewPa
 uint32_t project_blocks(uint32_t n) { return n ? (n + PROJECT_THREADS - 1) / PROJECT_THREADS : 1; }
 
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
-                          uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid) {
+                          uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid,
+                          uint32_t *counters) {
     if (sc.n == 0 || nviews == 0) return hipSuccess;
     const uint32_t grid = project_blocks(sc.n);
-    const ProjectOut o{rec, sbox, full, dbg, dbg_bbox, dbg_valid};
+    const ProjectOut o{rec, sbox, full, dbg, dbg_bbox, dbg_valid, counters};
     switch (sc.storage) {
     case SP_STORE_F16: launch_store<SP_STORE_F16>(s, grid, sc, views, nviews, o); break;
     case SP_STORE_BF16: launch_store<SP_STORE_BF16>(s, grid, sc, views, nviews, o); break;

@@ -50,6 +50,7 @@ enum Counter : int {
     CNT_STAGED = 3,    // pair records the blends gathered (early termination skips the rest)
     CNT_PAIRS2 = 4,    // pairs of the second emission phase
     CNT_SBPAIRS = 5,   // (superblock, splat) pairs of the binned first phase
+    CNT_DEPTH_RANGE = 6,  // set when a depth key reached 2^DEPTH_KEY_BITS (never cleared on the device)
     CNT_SLOTS = 16
 };
 
@@ -75,7 +76,11 @@ struct ViewCtx {
     float focal, cx, cy, width, height, near_z, lim_x, lim_y;
     uint32_t w, h, tiles_x, tiles_y;
     float bg[3];
-    uint32_t pad;
+    // Depth-sort key base: when nonzero, a visible splat's key is its depth
+    // bits less key_base, below 2^DEPTH_KEY_BITS (the host bounds the depth
+    // range of the batch's views from the scene's bounding sphere); 0 keeps
+    // the full 32-bit depth bits.
+    uint32_t key_base;
 };
 
 // One view of a batch, in device memory: its context, the first global tile id
@@ -161,5 +166,9 @@ constexpr float NEAR_SINGULAR = 1e-3f;
 // Depth-key marker of a culled (view, splat) in the uncompacted key array;
 // visible depths are positive finite floats, whose bits never reach it.
 constexpr uint32_t FULL_CULLED = 0xFFFFFFFFu;
+// Width of the depth keys relative to ViewCtx::key_base: three 9-bit radix
+// passes instead of four 8-bit ones. Depth bits from just above the near
+// plane up to 2^27 higher span 16 binary orders of magnitude of depth.
+constexpr int DEPTH_KEY_BITS = 27;
 
 }  // namespace sp

@@ -34,11 +34,14 @@ hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 uint32_t project_blocks(uint32_t n);
 // Projects every splat into each of `nviews` views: records rec[3*(v*n+i)..]
 // for visible splats, their pixel boxes sbox[v*n + i], and full[v*n + i] =
-// the depth bits, or FULL_CULLED for a culled splat (the depth sort's sparse
-// first pass reads it directly). dbg / dbg_bbox / dbg_valid (optional, n
-// entries) get view 0's full per-splat projection.
+// the depth bits less the view's key_base, or FULL_CULLED for a culled splat
+// (the depth sort's sparse first pass reads it directly). A key that does not
+// fit DEPTH_KEY_BITS under a nonzero key_base sets counters[CNT_DEPTH_RANGE]
+// (counters may be null). dbg / dbg_bbox / dbg_valid (optional, n entries)
+// get view 0's full per-splat projection.
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
-                          uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid);
+                          uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid,
+                          uint32_t *counters);
 
 // ---- prep.hip: scene layout on the device ----
 // The caller's arrays of structures, on the device: positions [n][3], cov3d

@@ -463,6 +463,50 @@ def test_degenerate_inputs_match_cpu(oracle):
     r.close()
 
 
+def test_depth_key_width_follows_scene_bound(oracle):
+    """The depth sort's keys are the depth bits less the near plane's bits, 27
+    bits wide, when the scene's bounding sphere keeps every depth of the view
+    within 2^27 of them (three 9-bit passes), and the full 32 bits otherwise
+    (a near plane so close that the far side of the scene lies more than 16
+    binary orders of magnitude beyond it, or far-flung splats). Either way the
+    key lists (reported with the depth bits restored), tile ranges and image
+    bytes equal the cpu restatement's, also in a batch that mixes both kinds of
+    view (which then sorts full keys)."""
+    s = scene("random20k")
+    r = S.HipRenderer(s)
+    cpu = oracle.CpuTiled(s)
+    near_views = _views(s)
+    eye = np.asarray(near_views[0].eye, np.float32)
+    far_near = S.view_of(eye, near_views[0].right, near_views[0].down, near_views[0].forward,
+                         near_views[0].fov_y, 1e-6)
+    for vi, (v, bits) in enumerate([(near_views[0], 27), (near_views[1], 27), (far_near, 32)]):
+        for (w, h) in [(320, 180), (97, 53)]:
+            case(f"depth key width view {vi} {w}x{h}")
+            gk, gv, gt = r.debug_pairs(v, w, h)
+            assert r.frame_info()["depth_key_bits"] == bits
+            ok, ov, ot = cpu.pairs(v, w, h)
+            assert np.array_equal(gt, ot) and np.array_equal(gk, ok) and np.array_equal(gv, ov)
+            ref, _ = cpu.render(v, w, h, BG)
+            img, _ = r.render(v, w, h, BG)
+            assert r.frame_info()["depth_key_bits"] == bits
+            assert np.array_equal(img.pixels, ref), f"{np.count_nonzero(img.pixels != ref)} bytes differ"
+    case("depth key width: mixed batch")
+    views = [near_views[0], far_near, near_views[2]]
+    sizes = [(320, 180), (200, 120), (97, 53)]
+    imgs, _ = r.render_batch(views, sizes, BG)
+    assert r.frame_info()["depth_key_bits"] == 32
+    for v, (w, h), img in zip(views, sizes, imgs):
+        ref, _ = cpu.render(v, w, h, BG)
+        assert np.array_equal(img, ref)
+    r.close()
+    case("depth key width: far-flung splats")
+    d = _degenerate_scene(6000, 43)
+    rd = S.HipRenderer(d)
+    rd.render(_views(d)[0], 160, 90, BG)
+    assert rd.frame_info()["depth_key_bits"] == 32
+    rd.close()
+
+
 def test_batch_of_degenerate_scene_and_empty_views(oracle):
     """The degenerate scene through one batch: random views, a view with the
     scene behind it (only far-flung splats of the 1e30-scaled group in front),
