@@ -154,7 +154,8 @@ struct sp_handle {
     uint64_t generation = 1;  // bumped whenever a device buffer moves
     float *d_soa = nullptr;
     uint16_t *d_soa16 = nullptr;
-    uint64_t soa_bytes = 0, soa16_bytes = 0;
+    void *d_crow = nullptr;  // colour rows (f32 or 16-bit entries, sp::colour_row per splat)
+    uint64_t soa_bytes = 0, soa16_bytes = 0, crow_bytes = 0;
     // The scene may give a splat an infinite colour (launch_prep's flag): its
     // frames blend through the instance that reproduces the reference's
     // NaN channels beside such splats.
@@ -572,6 +573,9 @@ sp::DevScene dev_scene(const sp_handle *h) {
     sp::DevScene s;
     s.soa = h->d_soa;
     s.soa16 = h->d_soa16;
+    const bool half = h->opt.storage != SP_STORE_F32;
+    s.crow = half ? nullptr : static_cast<const float *>(h->d_crow);
+    s.crow16 = half ? static_cast<const uint16_t *>(h->d_crow) : nullptr;
     s.stride = h->stride;
     s.n = h->n;
     s.deg = h->deg;
@@ -614,6 +618,12 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     const sp::EmitSplit es = split ? sp::EmitSplit{h->split_kmin, h->split_shift} : sp::EmitSplit{0u, 0u};
     sp::PairSlots slots{h->d_pbase, h->d_pslot, h->d_prid};
     const sp::PairSlots *pslots = mode == BL_DIFF ? &slots : nullptr;
+    // A single view's stored-scene colours are evaluated by the emission, for
+    // the splats a blend reads; a batch's projection, bound by its record
+    // writes, evaluates them itself, as does the differentiable projection. A
+    // frame that stops before the blend needs none.
+    const bool defer = nviews <= sp::DEFER_COLOUR_VIEWS && sc.storage != sp::STORE_PARAMS;
+    uint4 *colour_rec = (defer && mode != BL_NONE) ? h->d_rec : nullptr;
     auto mark = [&](int k) { return events ? hipEventRecord(h->ev[k], s) : hipSuccess; };
     HIP_TRY(hipMemcpyAsync(h->d_views, h->h_views, nviews * sizeof(ViewParams), hipMemcpyHostToDevice, s));
     HIP_TRY(mark(0));
@@ -621,7 +631,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     // depth sort's sparse first pass compacts them (scene-index order within a
     // digit, so the sort stays stable) and writes each view's visible count.
     HIP_TRY(sp::launch_project(s, sc, h->d_views, nviews, h->d_rec, h->d_sbox, h->d_dkey[0], nullptr, nullptr,
-                               nullptr, h->d_counters));
+                               nullptr, h->d_counters, !defer));
     if (h->n == 0) HIP_TRY(sp::launch_fill(s, vis, nviews, 0u));  // no sort runs to count them
     int di = 0;
     HIP_TRY(sp::launch_radix_sort(s, h->d_dkey, h->d_dval, vis, nviews, h->n, h->n, h->depth_bits, h->d_rowtot,
@@ -633,7 +643,7 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
                             es, split ? h->d_openm : nullptr, split ? vopen : nullptr, h->d_bcnt, h->d_bcnt_all,
                             vtot, view_pairs,
                             h->d_bsb, vsb, h->d_pkey[0], h->d_pval[0], pslots,
-                            h->pair_cap, h->d_counters));
+                            h->pair_cap, h->d_counters, sc, colour_rec));
     HIP_TRY(mark(1));
     int pi = 0;
     {
@@ -667,7 +677,8 @@ int enqueue(sp_handle *h, uint32_t nviews, uint32_t total_tiles, uint32_t total_
     if (split) {
         HIP_TRY(sp::launch_emit(s, sp::EMIT_SECOND, h->d_dval[di], h->n, nviews, vis, h->d_sbox, h->d_box,
                                 h->d_views, es, h->d_openm, vopen, h->d_bcnt, h->d_bcnt_all, vtot, view_pairs, nullptr,
-                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters));
+                                nullptr, h->d_pkey[0], h->d_pval[0], nullptr, h->pair_cap, h->d_counters, sc,
+                                colour_rec));
     }
     HIP_TRY(mark(4));
     if (split) {
@@ -874,8 +885,10 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     const bool f32 = h->opt.storage == SP_STORE_F32;
     h->soa_bytes = (f32 ? (uint64_t)sp::A_SH + 3ull * rest : (uint64_t)sp::L_COLS) * S * 4;
     h->soa16_bytes = f32 ? 0 : ((uint64_t)sp::H_SH + 3ull * rest) * S * 2;
+    h->crow_bytes = (uint64_t)sp::colour_row(rest, !f32) * std::max<uint32_t>(n, 1) * (f32 ? 4 : 2);
     int rc = dalloc(h, (void **)&h->d_soa, h->soa_bytes);
     if (!rc && !f32) rc = dalloc(h, (void **)&h->d_soa16, h->soa16_bytes);
+    if (!rc) rc = dalloc(h, &h->d_crow, h->crow_bytes);
     if (rc) return rc;
     // The last block is padded to S splats: the padding reads as zero.
     HIP_TRY(hipMemsetAsync(h->d_soa, 0, h->soa_bytes, h->stream));
@@ -958,7 +971,9 @@ int upload_scene(sp_handle *h, const sp_scene *scene) {
     if (!rc && hipMemcpyAsync(dk, kv.data(), 4ull * n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = SP_E_DEVICE;
     hipError_t e = rc ? hipErrorUnknown
-                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, h->d_soa, h->d_soa16, d_flag);
+                      : sp::launch_prep(h->stream, h->opt.storage, in, n, rest, h->d_soa, h->d_soa16,
+                                        f32 ? static_cast<float *>(h->d_crow) : nullptr,
+                                        f32 ? nullptr : static_cast<uint16_t *>(h->d_crow), d_flag);
     uint32_t flag = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1098,6 +1113,7 @@ void sp_destroy(sp_handle *h) {
     drop_graphs(h);
     (void)hipFree(h->d_soa);
     (void)hipFree(h->d_soa16);
+    (void)hipFree(h->d_crow);
     (void)hipFree(h->d_rec);
     (void)hipFree(h->d_box);
     (void)hipFree(h->d_sbox);
@@ -1368,7 +1384,7 @@ int sp_debug_project(sp_handle *h, const sp_view *view, uint32_t width, uint32_t
     if (e == hipSuccess) e = hipMemcpyAsync(h->d_views, h->h_views, sizeof(ViewParams), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
         e = sp::launch_project(h->stream, dev_scene(h), h->d_views, 1, h->d_rec, h->d_sbox, h->d_dkey[0], d_dbg, d_bb,
-                               d_valid, nullptr);
+                               d_valid, nullptr, false);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(proj, d_dbg, n * 48, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(bbox, d_bb, n * 8, hipMemcpyDeviceToHost);

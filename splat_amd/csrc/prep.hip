@@ -5,9 +5,11 @@
 // per splat. Each output column of a block is written by consecutive threads
 // (coalesced); each splat's own fields are read from its contiguous record.
 //
-// f32 storage: position x3, cov3d x6, opacity, k, base colour x3, SH rest.
-// 16-bit storage: f32 position x3, opacity, k; scale x3, quaternion x4, base
-// colour x3 and SH rest rounded to FP16 or BF16 (round to nearest even, the
+// f32 storage: columns position x3, cov3d x6, opacity, k, base colour x3, SH
+// rest; the colour again as rows.
+// 16-bit storage: f32 columns position x3, opacity, k; 16-bit columns scale
+// x3, quaternion x4, base colour x3 and SH rest; the colour again as 16-bit
+// rows; all 16-bit values rounded to FP16 or BF16 (round to nearest even, the
 // host's conversion code compiled for the device, so the bits are the host's).
 // k = sqrt(2 ln(o / ALPHA_MIN)) arrives precomputed by the host, whose logf
 // the reference's bits follow.
@@ -28,7 +30,7 @@ __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos,
                                                   const float *__restrict__ opac, const float *__restrict__ kk,
                                                   const float *__restrict__ col, const float *__restrict__ sh,
                                                   uint32_t n, uint32_t rest, float *__restrict__ soa,
-                                                  uint32_t *__restrict__ extreme) {
+                                                  float *__restrict__ crow, uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     bool bad = false;
     const uint32_t NC = A_SH + 3 * rest;
@@ -39,16 +41,19 @@ __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos,
         for (int c = 0; c < 6; ++c) soa[scene_at(NC, A_COV + c, i)] = cov[6ull * i + c];
         soa[scene_at(NC, A_OPAC, i)] = opac[i];
         soa[scene_at(NC, A_K, i)] = kk[i];
+        float *row = crow + (uint64_t)colour_row(rest, false) * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < 1e36f);
             soa[scene_at(NC, A_COL + c, i)] = v;
+            row[c] = v;
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < 1e36f);
             soa[scene_at(NC, A_SH + c, i)] = v;
+            row[3 + c] = v;
         }
     }
     flag_extreme(bad, extreme);
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
                                                  const float *__restrict__ kk, const float *__restrict__ col,
                                                  const float *__restrict__ sh, uint32_t n, uint32_t rest,
                                                  float *__restrict__ soa, uint16_t *__restrict__ soa16,
-                                                 uint32_t *__restrict__ extreme) {
+                                                 uint16_t *__restrict__ crow16, uint32_t *__restrict__ extreme) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const float lim = BF16 ? 1e36f : 65504.0f;
     bool bad = false;
@@ -79,16 +84,17 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
         for (int c = 0; c < 3; ++c) soa16[scene_at(NH, H_SCALE + c, i)] = to16<BF16>(scl[3ull * i + c]);
 #pragma unroll
         for (int c = 0; c < 4; ++c) soa16[scene_at(NH, H_QUAT + c, i)] = to16<BF16>(quat[4ull * i + c]);
+        uint16_t *row = crow16 + (uint64_t)colour_row(rest, true) * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[scene_at(NH, H_COL + c, i)] = to16<BF16>(v);
+            soa16[scene_at(NH, H_COL + c, i)] = row[c] = to16<BF16>(v);
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[scene_at(NH, H_SH + c, i)] = to16<BF16>(v);
+            soa16[scene_at(NH, H_SH + c, i)] = row[3 + c] = to16<BF16>(v);
         }
     }
     flag_extreme(bad, extreme);
@@ -97,18 +103,18 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
 }  // namespace
 
 hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, float *soa,
-                       uint16_t *soa16, uint32_t *extreme) {
+                       uint16_t *soa16, float *crow, uint16_t *crow16, uint32_t *extreme) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (n + 255) / 256;
     if (storage == SP_STORE_F32)
         hipLaunchKernelGGL(k_prep_f32, dim3(grid), dim3(256), 0, s, in.pos, in.cov, in.opac, in.k, in.col, in.sh, n,
-                           rest, soa, extreme);
+                           rest, soa, crow, extreme);
     else if (storage == SP_STORE_F16)
         hipLaunchKernelGGL(k_prep_16<false>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, soa, soa16, extreme);
+                           in.col, in.sh, n, rest, soa, soa16, crow16, extreme);
     else
         hipLaunchKernelGGL(k_prep_16<true>, dim3(grid), dim3(256), 0, s, in.pos, in.scl, in.quat, in.opac, in.k,
-                           in.col, in.sh, n, rest, soa, soa16, extreme);
+                           in.col, in.sh, n, rest, soa, soa16, crow16, extreme);
     return hipGetLastError();
 }
 

@@ -109,19 +109,27 @@ struct ViewParams {
 // xx,xy,xz,yy,yz,zz; 9 opacity; 10 k = sqrt(2 ln(o/ALPHA_MIN)); 11-13 base
 // colour; 14.. SH rest coefficient c channel ch at 14 + 3c + ch.
 //
+// The colour is also stored as one contiguous row per splat (colour_row
+// entries: base colour, then SH rest coefficient c channel ch at 3 + 3c + ch,
+// then padding). A single view's projection reads no colour (the columns
+// above up to k only), and the emission evaluates the colour of just the
+// splats whose pairs it emits for a blend, each row one or two 128-byte lines
+// of a gather. A batch's projection, whose time goes to writing its views'
+// records, evaluates the colour from the columns as it projects.
+//
 // Reduced-precision storage (FP16 / BF16) keeps the f32 columns 0-2 position,
 // 3 opacity, 4 k (a layout of L_COLS columns), and stores the covariance
 // factors and colour as 16-bit columns in soa16: 0-2 scale, 3-6 quaternion
 // x,y,z,w, 7-9 base colour, 10.. SH rest (coefficient c channel ch at
-// 10 + 3c + ch), blocked the same way. The covariance is rebuilt in f32 on
-// the device.
+// 10 + 3c + ch), blocked the same way, and its colour rows in 16 bits. The
+// covariance is rebuilt in f32 on the device.
 //
 // The differentiable path (storage STORE_PARAMS) reads raw, unactivated
 // parameters instead, row-major per gaussian as the caller's tensors hold
 // them: means [n][3], log-scales [n][3], quaternions [n][4] (x,y,z,w,
 // normalised on the fly), opacity logits [n], base colours [n][3], SH rest
 // [n][rest][3]. Activation (exp, sigmoid, normalisation, covariance, k) is
-// computed per use.
+// computed per use, and the projection evaluates the colour itself.
 constexpr uint32_t SCENE_BLOCK = 64;
 __host__ __device__ constexpr uint64_t scene_at(uint32_t ncols, uint32_t c, uint32_t i) {
     return ((uint64_t)(i / SCENE_BLOCK) * ncols + c) * SCENE_BLOCK + (i % SCENE_BLOCK);
@@ -129,9 +137,15 @@ __host__ __device__ constexpr uint64_t scene_at(uint32_t ncols, uint32_t c, uint
 struct DevParams {
     const float *means, *log_scales, *quats, *logits, *colors, *sh_rest;
 };
+// Entries of a splat's colour row: 3 + 3 * rest, rounded up to 16 bytes.
+__host__ __device__ constexpr uint32_t colour_row(uint32_t rest, bool half) {
+    return half ? (3u + 3u * rest + 7u) & ~7u : (3u + 3u * rest + 3u) & ~3u;
+}
 struct DevScene {
     const float *soa;
     const uint16_t *soa16;
+    const float *crow;      // f32 storage: colour rows
+    const uint16_t *crow16; // 16-bit storage: colour rows
     uint64_t stride;
     uint32_t n;
     uint32_t deg;
@@ -150,6 +164,8 @@ constexpr int NG3D_BASE = 14;
 constexpr int A_POS = 0, A_COV = 3, A_OPAC = 9, A_K = 10, A_COL = 11, A_SH = 14;
 constexpr int L_POS = 0, L_OPAC = 3, L_K = 4, L_COLS = 5;                // f32 arrays, 16-bit storage
 constexpr int H_SCALE = 0, H_QUAT = 3, H_COL = 7, H_SH = 10;             // 16-bit arrays
+// Batches of at most this many views leave the colour to the emission.
+constexpr uint32_t DEFER_COLOUR_VIEWS = 1;
 // det <= NEAR_SINGULAR * a * c marks a projected covariance whose f32
 // determinant lost more than ~10 of its 24 bits to cancellation; the 16-bit
 // storage paths recompute such determinants in FP64.

@@ -33,7 +33,9 @@ hipError_t launch_fill(hipStream_t s, uint32_t *p, uint64_t n, uint32_t value);
 // ---- project.hip: (a) projection + SH colour ----
 uint32_t project_blocks(uint32_t n);
 // Projects every splat into each of `nviews` views: records rec[3*(v*n+i)..]
-// for visible splats, their pixel boxes sbox[v*n + i], and full[v*n + i] =
+// for visible splats (with their colours when `colour` is set or the scene is
+// STORE_PARAMS; otherwise the colours stay zero, for the emission to fill in
+// those a blend reads, except in the debug output, which always gets them), their pixel boxes sbox[v*n + i], and full[v*n + i] =
 // the depth bits less the view's key_base, or FULL_CULLED for a culled splat
 // (the depth sort's sparse first pass reads it directly). A key that does not
 // fit DEPTH_KEY_BITS under a nonzero key_base sets counters[CNT_DEPTH_RANGE]
@@ -41,7 +43,7 @@ uint32_t project_blocks(uint32_t n);
 // get view 0's full per-splat projection.
 hipError_t launch_project(hipStream_t s, const DevScene &sc, const ViewParams *views, uint32_t nviews, uint4 *rec,
                           uint2 *sbox, uint32_t *full, float *dbg, uint16_t *dbg_bbox, uint8_t *dbg_valid,
-                          uint32_t *counters);
+                          uint32_t *counters, bool colour);
 
 // ---- prep.hip: scene layout on the device ----
 // The caller's arrays of structures, on the device: positions [n][3], cov3d
@@ -51,13 +53,14 @@ struct PrepInput {
     const float *pos, *cov, *scl, *quat, *opac, *k, *col, *sh;
 };
 // Writes the scene's blocked columns (scene_at) in `storage`'s layout: soa
-// (f32 columns) and, for 16-bit storage, soa16. Sets *extreme
+// (f32 columns) and, for 16-bit storage, soa16; and its colour rows (crow,
+// or crow16 for 16-bit storage, colour_row entries per splat). Sets *extreme
 // (cleared by the caller) when a base colour or SH coefficient is NaN,
 // infinite or too large for a colour sum to stay finite: at least 1e36 (f32,
 // BF16), or at least 65504 (FP16, where it may round to infinity). Otherwise
 // |colour| <= 1e36 * (1 + 15 * 1.1) stays below FLT_MAX at every direction.
 hipError_t launch_prep(hipStream_t s, int storage, const PrepInput &in, uint32_t n, uint32_t rest, float *soa,
-                       uint16_t *soa16, uint32_t *extreme);
+                       uint16_t *soa16, float *crow, uint16_t *crow16, uint32_t *extreme);
 
 // ---- radix.hip: (c) stable LSD radix sort ----
 // Words of the per-block digit-count table for nseg segments of up to n_cap
@@ -101,7 +104,10 @@ uint32_t emit_blocks(uint32_t n);
 // instead, counted in counters[CNT_SBPAIRS]; counters[CNT_PAIRS] still counts
 // the tile pairs. With slots (FIRST only), the emission slots of the tile
 // pairs too, numbered view by view, splat by splat in depth order, each
-// splat's rectangle row-major (see PairSlots).
+// splat's rectangle row-major (see PairSlots). With colour_rec, the phase
+// also writes the colour of each splat whose pairs it emits into its record
+// (sc's colour rows; FIRST: the phase's splats, SECOND: those with pairs in
+// unfinished tiles), which a stored scene's projection leaves out.
 struct PairSlots {
     uint32_t *pbase;  // views x n: a record's first slot (records with pairs)
     uint32_t *pslot;  // views x (n + 1): view v's j-th depth-ordered splat's first slot at v * (n + 1) + j,
@@ -113,7 +119,7 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
                        EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, const PairSlots *slots, uint64_t pair_cap,
-                       uint32_t *counters);
+                       uint32_t *counters, const DevScene &sc, uint4 *colour_rec);
 // Rows of the binned tile-count scan: tcount holds bin_scan_rows * BIN_SCAN_ROW
 // entries, trow bin_scan_rows.
 uint32_t bin_scan_rows(uint32_t total_tiles);

@@ -21,14 +21,16 @@ constexpr int rest_of() {
     return (DEG + 1) * (DEG + 1) - 1;
 }
 
-// colour += eval_rest(deg, rest, dir); sh holds coefficient k channel ch at 3k+ch.
-template <int DEG>
-__device__ __forceinline__ void sh_color(const float *sh, float x, float y, float z, float c[3]) {
-    auto r = [&](int k, int ch) { return sh[3 * k + ch]; };
+// colour += eval_rest(deg, rest, dir), with r(k, ch) the SH rest coefficient
+// k of channel ch. deg may be a run-time value; with a constant the compiler
+// folds the tests, and either way the operations are the same.
+template <typename R>
+__device__ __forceinline__ void sh_color_impl(int deg, const R &r, float x, float y, float z, float c[3]) {
+    if (deg < 1) return;
     float e[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) e[ch] = SH_C1 * (((-y) * r(0, ch) + z * r(1, ch)) - x * r(2, ch));
-    if (DEG >= 2) {
+    if (deg >= 2) {
         const float xx = x * x, yy = y * y, zz = z * z;
         const float xy = x * y, yz = y * z, xz = x * z;
         const float b3 = SH_C2[0] * xy, b4 = SH_C2[1] * yz, b5 = SH_C2[2] * ((2.0f * zz - xx) - yy);
@@ -42,7 +44,7 @@ __device__ __forceinline__ void sh_color(const float *sh, float x, float y, floa
             s = s + b7 * r(7, ch);
             e[ch] = e[ch] + s;
         }
-        if (DEG >= 3) {
+        if (deg >= 3) {
             const float b8 = (SH_C3[0] * y) * (3.0f * xx - yy);
             const float b9 = (SH_C3[1] * xy) * z;
             const float b10 = (SH_C3[2] * y) * ((4.0f * zz - xx) - yy);
@@ -65,6 +67,75 @@ __device__ __forceinline__ void sh_color(const float *sh, float x, float y, floa
     }
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + e[ch];
+}
+
+// colour += eval_rest(deg, rest, dir); sh holds coefficient k channel ch at 3k+ch.
+template <int DEG>
+__device__ __forceinline__ void sh_color(const float *sh, float x, float y, float z, float c[3]) {
+    sh_color_impl(DEG, [&](int k, int ch) { return sh[3 * k + ch]; }, x, y, z, c);
+}
+
+// The splat's view direction as render::project takes it: normalize_or_zero
+// of d = p - eye (glam: d * (1/|d|) when that is finite and > 0, else 0).
+__device__ __forceinline__ void view_dir(float d0, float d1, float d2, float u[3]) {
+    const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+    const float rcp = 1.0f / len;
+    u[0] = u[1] = u[2] = 0.0f;
+    if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+        u[0] = d0 * rcp;
+        u[1] = d1 * rcp;
+        u[2] = d2 * rcp;
+    }
+}
+
+// The colour of scene splat i seen from `eye` (f32 or 16-bit storage): the
+// base colour plus the SH rest at the view direction, clamped at 0, as
+// render::project computes it (src/render/mod.rs:250-262), from the splat's
+// colour row (read in 16-byte pieces: a gather of one row per lane) and its
+// position.
+__device__ __forceinline__ void row_colour(const DevScene &sc, uint32_t i, const float eye[3], float col[3]) {
+    constexpr uint32_t MAXW = 48;  // colour_row of SH degree 3 (f32 and 16-bit)
+    const bool half = sc.storage != SP_STORE_F32;
+    const uint32_t nc = A_SH + 3 * sc.rest;
+    float p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        p[c] = half ? sc.soa[scene_at(L_COLS, L_POS + c, i)] : sc.soa[scene_at(nc, A_POS + c, i)];
+    const uint32_t cw = colour_row(sc.rest, half);
+    float v[MAXW];
+    if (!half) {
+        const float4 *row = reinterpret_cast<const float4 *>(sc.crow + (uint64_t)cw * i);
+#pragma unroll
+        for (uint32_t q = 0; q < MAXW / 4; ++q) {
+            const float4 x = 4 * q < cw ? row[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+        const uint4 *row = reinterpret_cast<const uint4 *>(sc.crow16 + (uint64_t)cw * i);
+        const bool bf16 = sc.storage == SP_STORE_BF16;
+#pragma unroll
+        for (uint32_t q = 0; q < MAXW / 8; ++q) {
+            const uint4 x = 8 * q < cw ? row[q] : make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint16_t hv = (uint16_t)(w[h >> 1] >> (16 * (h & 1)));
+                v[8 * q + h] = bf16 ? __uint_as_float((uint32_t)hv << 16) : (float)__builtin_bit_cast(_Float16, hv);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[c] = v[c];
+    if (sc.deg > 0) {
+        float u[3];
+        view_dir(p[0] - eye[0], p[1] - eye[1], p[2] - eye[2], u);
+        sh_color_impl((int)sc.deg, [&](int k, int ch) { return v[3 + 3 * k + ch]; }, u[0], u[1], u[2], col);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
 }
 
 // A splat's parameters, loaded once.
@@ -127,7 +198,8 @@ __device__ __forceinline__ float k_of_opacity(float o) {
     return o < ALPHA_MIN ? 0.0f : sqrtf(2.0f * logf(o / ALPHA_MIN));
 }
 
-template <int DEG, int STORE>
+// COLOUR: also the base colour and SH rest (always for STORE_PARAMS).
+template <int DEG, int STORE, bool COLOUR = true>
 __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat<DEG> &s) {
     const float *A = sc.soa;
     if (STORE == STORE_PARAMS) {
@@ -156,10 +228,12 @@ __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat
         for (int c = 0; c < 3; ++c) s.p[c] = A[scene_at(NC, A_POS + c, i)];
 #pragma unroll
         for (int c = 0; c < 6; ++c) s.cv[c] = A[scene_at(NC, A_COV + c, i)];
+        if constexpr (COLOUR) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.col[c] = A[scene_at(NC, A_COL + c, i)];
+            for (int c = 0; c < 3; ++c) s.col[c] = A[scene_at(NC, A_COL + c, i)];
 #pragma unroll
-        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = A[scene_at(NC, A_SH + c, i)];
+            for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = A[scene_at(NC, A_SH + c, i)];
+        }
     } else {
         const uint16_t *B = sc.soa16;
         constexpr uint32_t NH = H_SH + 3 * rest_of<DEG>();
@@ -173,18 +247,36 @@ __device__ __forceinline__ void load_splat(const DevScene &sc, uint32_t i, Splat
 #pragma unroll
         for (int c = 0; c < 4; ++c) q[c] = h2f<STORE>(B[scene_at(NH, H_QUAT + c, i)]);
         covariance_f32(scl, q, s.cv);
+        if constexpr (COLOUR) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s.col[c] = h2f<STORE>(B[scene_at(NH, H_COL + c, i)]);
+            for (int c = 0; c < 3; ++c) s.col[c] = h2f<STORE>(B[scene_at(NH, H_COL + c, i)]);
 #pragma unroll
-        for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = h2f<STORE>(B[scene_at(NH, H_SH + c, i)]);
+            for (int c = 0; c < 3 * rest_of<DEG>(); ++c) s.sh[c] = h2f<STORE>(B[scene_at(NH, H_SH + c, i)]);
+        }
     }
 }
 
+// The clamped colour of a loaded splat (base + SH at the view direction).
+template <int DEG>
+__device__ __forceinline__ void splat_colour(const Splat<DEG> &s, float d0, float d1, float d2, float col[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[c] = s.col[c];
+    if (DEG > 0) {
+        float u[3];
+        view_dir(d0, d1, d2, u);
+        sh_color<DEG>(s.sh, u[0], u[1], u[2], col);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
+}
+
 // render::project of one splat into one view. Writes the 48-byte record and
-// returns true when the splat survives culling.
-template <int DEG, bool PREC>
+// returns true when the splat survives culling. colour(d0, d1, d2, col) gives
+// the splat's clamped colour for d = p - eye (it may leave zeros where the
+// emission fills the colour in later).
+template <int DEG, bool PREC, typename COL>
 __device__ __forceinline__ bool project_view(const Splat<DEG> &s, const ViewCtx &v, uint4 *o, float *depth,
-                                             float *dbg, uint16_t *dbg_bbox, uint2 *box = nullptr) {
+                                             float *dbg, uint16_t *dbg_bbox, uint2 *box, const COL &colour) {
     const float d0 = s.p[0] - v.eye[0], d1 = s.p[1] - v.eye[1], d2 = s.p[2] - v.eye[2];
     // t = rot * (p - eye), glam column order.
     const float tx = (v.rot[0][0] * d0 + v.rot[0][1] * d1) + v.rot[0][2] * d2;
@@ -235,21 +327,8 @@ __device__ __forceinline__ bool project_view(const Splat<DEG> &s, const ViewCtx 
     ok = ok && __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(ext_x) &&
          __builtin_isfinite(ext_y);
     if (!ok) return false;
-    float col[3] = {s.col[0], s.col[1], s.col[2]};
-    if (DEG > 0) {
-        // normalize_or_zero(p - eye) (glam): d * (1/|d|) when that is finite and > 0.
-        const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-        const float rcp = 1.0f / len;
-        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-        if (__builtin_isfinite(rcp) && rcp > 0.0f) {
-            ux = d0 * rcp;
-            uy = d1 * rcp;
-            uz = d2 * rcp;
-        }
-        sh_color<DEG>(s.sh, ux, uy, uz, col);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
+    float col[3];
+    colour(d0, d1, d2, col);
     const float ca = cc * inv_det, cb = -b * inv_det, cc2 = a * inv_det;
     // cpu::pixel_bbox (src/render/cpu.rs:96-102).
     const uint32_t bx0 = sat_u32(fmaxf(floorf(x - ext_x), 0.0f));

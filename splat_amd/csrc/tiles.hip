@@ -42,6 +42,7 @@
 
 #include "sp_device.h"
 #include "sp_launch.h"
+#include "sp_splat.h"
 
 namespace sp {
 namespace {
@@ -190,6 +191,19 @@ __device__ __forceinline__ uint32_t phase_count(const Rect &r, uint32_t j, uint3
     return j < k1 ? r.cnt : 0u;
 }
 
+// The colour of view v's record of scene splat i, written into the record
+// (a stored scene's projection leaves it to the phase whose pairs first
+// include the splat, so only splats that a blend reads are evaluated).
+__device__ __forceinline__ void emit_colour(const DevScene &sc, const ViewCtx &vc, uint4 *rec, uint64_t rid,
+                                            uint32_t i) {
+    float col[3];
+    row_colour(sc, i, vc.eye, col);
+    float *rf = reinterpret_cast<float *>(rec + 3 * rid);
+    rf[6] = col[0];  // record word 1: x, y, z = red, w = green
+    rf[7] = col[1];
+    rf[8] = col[2];  // record word 2: x = blue
+}
+
 template <int PHASE>
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__restrict__ dval, uint32_t n_splats,
                                                              uint32_t bpv, const uint32_t *__restrict__ vis_count,
@@ -199,7 +213,8 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
                                                              uint32_t *__restrict__ bcnt,
                                                              uint32_t *__restrict__ bcnt_all,
                                                              uint32_t *__restrict__ bsb,
-                                                             uint32_t *__restrict__ vopen) {
+                                                             uint32_t *__restrict__ vopen, DevScene sc,
+                                                             uint4 *__restrict__ rec) {
     const uint32_t v = blockIdx.x / bpv, local = blockIdx.x % bpv;
     if (PHASE == EMIT_SECOND && vopen[v] == 0) {  // block-uniform: the first phase finished the view
         if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
@@ -233,6 +248,12 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_count(const uint32_t *__r
         r = splat_rect<false>(dval, box, vbase, j, nvis, tiles_x, tile_base);
     }
     const uint32_t c = phase_count<PHASE>(r, j, k1, openm, tile_base, tiles_x);
+    // Colours of the splats whose pairs this phase emits (rec is null where
+    // the projection wrote them).
+    if (rec && j < nvis && (PHASE == EMIT_FIRST ? j < k1 : c > 0u)) {
+        const uint32_t i = dval[vbase + j];
+        emit_colour(sc, views[v].ctx, rec, vbase + i, i);
+    }
     if (PHASE == EMIT_SECOND) {
         const uint32_t v1[1] = {c};
         uint32_t t1[1];
@@ -764,11 +785,13 @@ hipError_t emit_phase(hipStream_t s, const uint32_t *dval, uint32_t n_splats, ui
                       EmitSplit split,
                       uint32_t *openm, uint32_t *bcnt, uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs,
                       uint32_t *bsb, uint32_t *vsb, uint32_t *vopen, uint32_t *pkey, uint32_t *pval,
-                      const PairSlots *slots, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters) {
+                      const PairSlots *slots, uint64_t pair_cap, uint32_t count_slot, uint32_t *counters,
+                      const DevScene &sc, uint4 *colour_rec) {
     const uint32_t bpv = emit_blocks(n_splats);
     const bool bin = PHASE == EMIT_FIRST && bsb != nullptr;
     hipLaunchKernelGGL(k_emit_count<PHASE>, dim3(bpv * nviews), dim3(EMIT_THREADS), 0, s, dval, n_splats, bpv,
-                       vis_count, sbox, box, views, split, openm, bcnt, bcnt_all, bin ? bsb : nullptr, vopen);
+                       vis_count, sbox, box, views, split, openm, bcnt, bcnt_all, bin ? bsb : nullptr, vopen, sc,
+                       colour_rec);
     hipError_t e = hipGetLastError();
     // Per view: block offsets and the view's pair total (and, in FIRST, the
     // unsplit pair count of every view, RenderStats.overlaps, and the binned
@@ -813,15 +836,15 @@ hipError_t launch_emit(hipStream_t s, int phase, const uint32_t *dval, uint32_t 
                        EmitSplit split, uint32_t *openm, uint32_t *vopen, uint32_t *bcnt,
                        uint32_t *bcnt_all, uint32_t *vtot, uint32_t *view_pairs, uint32_t *bsb, uint32_t *vsb,
                        uint32_t *pkey, uint32_t *pval, const PairSlots *slots, uint64_t pair_cap,
-                       uint32_t *counters) {
+                       uint32_t *counters, const DevScene &sc, uint4 *colour_rec) {
     if (phase == EMIT_SECOND)
         return emit_phase<EMIT_SECOND>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt,
                                        bcnt_all, vtot, view_pairs, nullptr, nullptr, vopen, pkey, pval, nullptr,
-                                       pair_cap, CNT_PAIRS2, counters);
+                                       pair_cap, CNT_PAIRS2, counters, sc, colour_rec);
     if (!bsb || !vsb) return hipErrorInvalidValue;  // the first phase is always binned
     return emit_phase<EMIT_FIRST>(s, dval, n_splats, nviews, vis_count, sbox, box, views, split, openm, bcnt, bcnt_all,
                                   vtot, view_pairs, bsb, vsb, vopen, pkey, pval, slots, pair_cap, CNT_PAIRS,
-                                  counters);
+                                  counters, sc, colour_rec);
 }
 
 hipError_t launch_bin_lists(hipStream_t s, const ViewParams *views, uint32_t nviews, uint32_t total_sbs,
