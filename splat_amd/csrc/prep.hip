@@ -41,20 +41,24 @@ __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos,
         for (int c = 0; c < 6; ++c) soa[scene_at(NC, A_COV + c, i)] = cov[6ull * i + c];
         soa[scene_at(NC, A_OPAC, i)] = opac[i];
         soa[scene_at(NC, A_K, i)] = kk[i];
-        float *row = crow + (uint64_t)colour_row(rest, false) * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < 1e36f);
             soa[scene_at(NC, A_COL + c, i)] = v;
-            row[c] = v;
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < 1e36f);
             soa[scene_at(NC, A_SH + c, i)] = v;
-            row[3 + c] = v;
         }
+        // The row in 16-byte stores: base colour, SH rest, zero padding.
+        const uint32_t cw = colour_row(rest, false);
+        const auto val = [&](uint32_t q) {
+            return q < 3u ? col[3ull * i + q] : q - 3u < 3u * rest ? sh[(uint64_t)3 * rest * i + (q - 3u)] : 0.0f;
+        };
+        float4 *row = reinterpret_cast<float4 *>(crow + (uint64_t)cw * i);
+        for (uint32_t q = 0; q < cw; q += 4) row[q / 4] = make_float4(val(q), val(q + 1), val(q + 2), val(q + 3));
     }
     flag_extreme(bad, extreme);
 }
@@ -84,18 +88,27 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
         for (int c = 0; c < 3; ++c) soa16[scene_at(NH, H_SCALE + c, i)] = to16<BF16>(scl[3ull * i + c]);
 #pragma unroll
         for (int c = 0; c < 4; ++c) soa16[scene_at(NH, H_QUAT + c, i)] = to16<BF16>(quat[4ull * i + c]);
-        uint16_t *row = crow16 + (uint64_t)colour_row(rest, true) * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float v = col[3ull * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[scene_at(NH, H_COL + c, i)] = row[c] = to16<BF16>(v);
+            soa16[scene_at(NH, H_COL + c, i)] = to16<BF16>(v);
         }
         for (uint32_t c = 0; c < 3 * rest; ++c) {
             const float v = sh[(uint64_t)3 * rest * i + c];
             bad |= !(fabsf(v) < lim);
-            soa16[scene_at(NH, H_SH + c, i)] = row[3 + c] = to16<BF16>(v);
+            soa16[scene_at(NH, H_SH + c, i)] = to16<BF16>(v);
         }
+        // The row in 16-byte stores of eight values: base colour, SH rest, zero padding.
+        const uint32_t cw = colour_row(rest, true);
+        const auto val = [&](uint32_t q) -> uint32_t {
+            return q < 3u ? to16<BF16>(col[3ull * i + q])
+                          : q - 3u < 3u * rest ? to16<BF16>(sh[(uint64_t)3 * rest * i + (q - 3u)]) : 0u;
+        };
+        uint4 *row = reinterpret_cast<uint4 *>(crow16 + (uint64_t)cw * i);
+        for (uint32_t q = 0; q < cw; q += 8)
+            row[q / 8] = make_uint4(val(q) | val(q + 1) << 16, val(q + 2) | val(q + 3) << 16,
+                                    val(q + 4) | val(q + 5) << 16, val(q + 6) | val(q + 7) << 16);
     }
     flag_extreme(bad, extreme);
 }
