@@ -52,10 +52,12 @@ __global__ __launch_bounds__(256) void k_prep_f32(const float *__restrict__ pos,
             bad |= !(fabsf(v) < 1e36f);
             soa[scene_at(NC, A_SH + c, i)] = v;
         }
-        // The row in 16-byte stores: base colour, SH rest, zero padding.
+        // The row in 16-byte stores: position, base colour, SH rest, zero padding.
         const uint32_t cw = colour_row(rest, false);
         const auto val = [&](uint32_t q) {
-            return q < 3u ? col[3ull * i + q] : q - 3u < 3u * rest ? sh[(uint64_t)3 * rest * i + (q - 3u)] : 0.0f;
+            return q < 3u   ? pos[3ull * i + q]
+                   : q < 6u ? col[3ull * i + (q - 3u)]
+                   : q - 6u < 3u * rest ? sh[(uint64_t)3 * rest * i + (q - 6u)] : 0.0f;
         };
         float4 *row = reinterpret_cast<float4 *>(crow + (uint64_t)cw * i);
         for (uint32_t q = 0; q < cw; q += 4) row[q / 4] = make_float4(val(q), val(q + 1), val(q + 2), val(q + 3));
@@ -99,11 +101,13 @@ __global__ __launch_bounds__(256) void k_prep_16(const float *__restrict__ pos, 
             bad |= !(fabsf(v) < lim);
             soa16[scene_at(NH, H_SH + c, i)] = to16<BF16>(v);
         }
-        // The row in 16-byte stores of eight values: base colour, SH rest, zero padding.
+        // The row in 16-byte stores of eight values: the f32 position's bits
+        // (low half first), base colour, SH rest, zero padding.
         const uint32_t cw = colour_row(rest, true);
         const auto val = [&](uint32_t q) -> uint32_t {
-            return q < 3u ? to16<BF16>(col[3ull * i + q])
-                          : q - 3u < 3u * rest ? to16<BF16>(sh[(uint64_t)3 * rest * i + (q - 3u)]) : 0u;
+            return q < 6u   ? (__float_as_uint(pos[3ull * i + q / 2u]) >> (16u * (q & 1u))) & 0xFFFFu
+                   : q < 9u ? to16<BF16>(col[3ull * i + (q - 6u)])
+                   : q - 9u < 3u * rest ? to16<BF16>(sh[(uint64_t)3 * rest * i + (q - 9u)]) : 0u;
         };
         uint4 *row = reinterpret_cast<uint4 *>(crow16 + (uint64_t)cw * i);
         for (uint32_t q = 0; q < cw; q += 8)

@@ -109,9 +109,10 @@ struct ViewParams {
 // xx,xy,xz,yy,yz,zz; 9 opacity; 10 k = sqrt(2 ln(o/ALPHA_MIN)); 11-13 base
 // colour; 14.. SH rest coefficient c channel ch at 14 + 3c + ch.
 //
-// The colour is also stored as one contiguous row per splat (colour_row
-// entries: base colour, then SH rest coefficient c channel ch at 3 + 3c + ch,
-// then padding). A single view's projection reads no colour (the columns
+// The colour is also stored as one contiguous row per splat with the
+// position it is seen from (colour_row entries: position x, y, z as f32 bits
+// (six entries of a 16-bit row), base colour, then SH rest coefficient c
+// channel ch, then padding). A single view's projection reads no colour (the columns
 // above up to k only), and the emission evaluates the colour of just the
 // splats whose pairs it emits for a blend, each row one or two 128-byte lines
 // of a gather. A batch's projection, whose time goes to writing its views'
@@ -137,9 +138,11 @@ __host__ __device__ constexpr uint64_t scene_at(uint32_t ncols, uint32_t c, uint
 struct DevParams {
     const float *means, *log_scales, *quats, *logits, *colors, *sh_rest;
 };
-// Entries of a splat's colour row: 3 + 3 * rest, rounded up to 16 bytes.
+// Entries of a splat's colour row (position, base colour, SH rest), rounded
+// up to 16 bytes; the base colour starts at entry colour_base.
+__host__ __device__ constexpr uint32_t colour_base(bool half) { return half ? 6u : 3u; }
 __host__ __device__ constexpr uint32_t colour_row(uint32_t rest, bool half) {
-    return half ? (3u + 3u * rest + 7u) & ~7u : (3u + 3u * rest + 3u) & ~3u;
+    return half ? (6u + 3u + 3u * rest + 7u) & ~7u : (3u + 3u + 3u * rest + 3u) & ~3u;
 }
 struct DevScene {
     const float *soa;

@@ -91,17 +91,13 @@ __device__ __forceinline__ void view_dir(float d0, float d1, float d2, float u[3
 // The colour of scene splat i seen from `eye` (f32 or 16-bit storage): the
 // base colour plus the SH rest at the view direction, clamped at 0, as
 // render::project computes it (src/render/mod.rs:250-262), from the splat's
-// colour row (read in 16-byte pieces: a gather of one row per lane) and its
-// position.
+// colour row, which holds its position too (read in 16-byte pieces: a gather
+// of one row per lane, two 128-byte lines).
 __device__ __forceinline__ void row_colour(const DevScene &sc, uint32_t i, const float eye[3], float col[3]) {
-    constexpr uint32_t MAXW = 48;  // colour_row of SH degree 3 (f32 and 16-bit)
+    constexpr uint32_t MAXW = 52;  // f32 colour_row of SH degree 3 (position 3, colour 3, rest 45, padding 1)
     const bool half = sc.storage != SP_STORE_F32;
-    const uint32_t nc = A_SH + 3 * sc.rest;
-    float p[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        p[c] = half ? sc.soa[scene_at(L_COLS, L_POS + c, i)] : sc.soa[scene_at(nc, A_POS + c, i)];
     const uint32_t cw = colour_row(sc.rest, half);
+    // v: position, base colour, SH rest as f32.
     float v[MAXW];
     if (!half) {
         const float4 *row = reinterpret_cast<const float4 *>(sc.crow + (uint64_t)cw * i);
@@ -114,25 +110,33 @@ __device__ __forceinline__ void row_colour(const DevScene &sc, uint32_t i, const
             v[4 * q + 3] = x.w;
         }
     } else {
+        // Entries: six halves of position bits, then the 16-bit values.
         const uint4 *row = reinterpret_cast<const uint4 *>(sc.crow16 + (uint64_t)cw * i);
         const bool bf16 = sc.storage == SP_STORE_BF16;
+        uint32_t w[(MAXW + 4) / 2];  // 56 halves as 28 words
 #pragma unroll
-        for (uint32_t q = 0; q < MAXW / 8; ++q) {
+        for (uint32_t q = 0; q < (MAXW + 4) / 8; ++q) {
             const uint4 x = 8 * q < cw ? row[q] : make_uint4(0u, 0u, 0u, 0u);
-            const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+            w[4 * q] = x.x;
+            w[4 * q + 1] = x.y;
+            w[4 * q + 2] = x.z;
+            w[4 * q + 3] = x.w;
+        }
 #pragma unroll
-            for (int h = 0; h < 8; ++h) {
-                const uint16_t hv = (uint16_t)(w[h >> 1] >> (16 * (h & 1)));
-                v[8 * q + h] = bf16 ? __uint_as_float((uint32_t)hv << 16) : (float)__builtin_bit_cast(_Float16, hv);
-            }
+        for (int c = 0; c < 3; ++c) v[c] = __uint_as_float(w[c]);
+#pragma unroll
+        for (uint32_t e = 3; e < MAXW; ++e) {
+            const uint32_t hidx = e + 3;  // half index in the row
+            const uint16_t hv = (uint16_t)(w[hidx >> 1] >> (16 * (hidx & 1)));
+            v[e] = bf16 ? __uint_as_float((uint32_t)hv << 16) : (float)__builtin_bit_cast(_Float16, hv);
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) col[c] = v[c];
+    for (int c = 0; c < 3; ++c) col[c] = v[3 + c];
     if (sc.deg > 0) {
         float u[3];
-        view_dir(p[0] - eye[0], p[1] - eye[1], p[2] - eye[2], u);
-        sh_color_impl((int)sc.deg, [&](int k, int ch) { return v[3 + 3 * k + ch]; }, u[0], u[1], u[2], col);
+        view_dir(v[0] - eye[0], v[1] - eye[1], v[2] - eye[2], u);
+        sh_color_impl((int)sc.deg, [&](int k, int ch) { return v[6 + 3 * k + ch]; }, u[0], u[1], u[2], col);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) col[c] = fmaxf(col[c], 0.0f);
