@@ -357,7 +357,7 @@ def bench_diff(S, torch, args) -> dict:
 def bench_precision(S, torch, args, W, H) -> dict:
     """3M SH-3 splats with the covariance factors (scale, rotation), colours and SH
     stored as f32 / FP16 / BF16: scene upload time, device bytes and views/s over
-    20 orbit views."""
+    the 40 orbit views after 20 warm-up views."""
     n = args.precision_splats
     log(f"precision: {n} splats")
     scene = S.random_scene(n, seed=1, sh_degree=3)
@@ -370,11 +370,11 @@ def bench_precision(S, torch, args, W, H) -> dict:
         t_up = time.perf_counter()
         r = S.HipRenderer(scene, storage=storage, timing=False)
         upload_ms = (time.perf_counter() - t_up) * 1e3
-        for i in range(3):
+        for i in range(20):
             r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        k = 20
+        k = 40
         for i in range(k):
             r.render_device(views[i], W, H, (0, 0, 0), dev.data_ptr())
         torch.cuda.synchronize()
