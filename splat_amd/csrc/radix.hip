@@ -4,8 +4,9 @@
 // Per digit (8 bits, 256 bins; or 9 bits, 512 bins, in a pass that saves a
 // pass, e.g. a 9-bit sort in one pass instead of 8 + 1):
 //   * upsweep: each block counts the digits of a tile (4096 keys, or 1024 for
-//     sorts small enough that 4096-key tiles would leave CUs idle) into a
-//     (segment, column, digit) table, a column being 1-8 consecutive tiles,
+//     sorts small enough that 4096-key tiles would leave CUs idle, 2048 in
+//     their 512-bin passes) into a (segment, column, digit) table, a column
+//     being 1-8 consecutive tiles,
 //   * digit-row scan: one workgroup per (segment, digit) row turns the counts
 //     of that digit across the columns into positions and the row's total,
 //   * downsweep: each block scans its segment's row totals into digit
@@ -334,9 +335,14 @@ namespace {
 // One pass: upsweep, digit-row scan, downsweep from keys[cur] / vals[cur] into the other buffers.
 template <int ITEMS, int BINS, bool SPARSE>
 hipError_t radix_pass(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], int cur, uint32_t *seg_counts,
-                      uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, uint32_t bps, uint32_t gps, int shift,
-                      int width, uint32_t *row_total, uint32_t *table) {
+                      uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int shift, int width, uint32_t *row_total,
+                      uint32_t *table) {
     constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
+    // Table columns per segment (at the capacity) and workgroups per segment:
+    // one per column up to a grid bound; they stride over the live columns, so
+    // a capacity far above the live count costs little.
+    const uint32_t bps = (uint32_t)((n_cap + TILE - 1) / TILE);
+    const uint32_t gps = std::max<uint32_t>(1, std::min<uint32_t>(bps, std::max<uint32_t>(1, RADIX_GRID / nseg)));
     hipLaunchKernelGGL((k_upsweep<ITEMS, BINS, SPARSE>), dim3(gps * nseg), dim3(RADIX_THREADS), 0, s, keys[cur],
                        seg_counts, seg_stride, n_cap, bps, gps, shift, width, table);
     hipError_t e = hipGetLastError();
@@ -353,14 +359,12 @@ template <int ITEMS>
 hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], uint32_t *seg_counts,
                         uint32_t nseg, uint64_t seg_stride, uint64_t n_cap, int bits, uint32_t *row_total,
                         uint32_t *table, int *out_index, bool sparse_first) {
-    constexpr uint32_t TILE = RADIX_THREADS * ITEMS;
+    // 512-bin passes of small sorts take 2048-key tiles (the table, sized for
+    // 1024-key tiles, has room for their columns).
+    constexpr int WIDE = ITEMS == RADIX_ITEMS_SMALL ? RADIX_ITEMS_WIDE : ITEMS;
     int widths[8];
     const int passes = radix_plan(bits, widths);
     int cur = 0;
-    const uint32_t bps = (uint32_t)((n_cap + TILE - 1) / TILE);
-    // Workgroups per segment: one per column up to a grid bound; they stride
-    // over the live columns, so a capacity far above the live count costs little.
-    const uint32_t gps = std::max<uint32_t>(1, std::min<uint32_t>(bps, std::max<uint32_t>(1, RADIX_GRID / nseg)));
     int shift = 0;
     for (int p = 0; p < passes; ++p) {
         const int width = widths[p];  // digit bits this pass carries
@@ -370,15 +374,15 @@ hipError_t radix_passes(hipStream_t s, uint32_t *keys[2], uint32_t *vals[2], uin
         // passes carry no sparse code.
         hipError_t e;
         if (bins == 512)
-            e = sparse ? radix_pass<ITEMS, 512, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps, gps,
-                                                       shift, width, row_total, table)
-                       : radix_pass<ITEMS, 512, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps,
-                                                        gps, shift, width, row_total, table);
+            e = sparse ? radix_pass<WIDE, 512, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, shift,
+                                                      width, row_total, table)
+                       : radix_pass<WIDE, 512, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, shift,
+                                                       width, row_total, table);
         else
-            e = sparse ? radix_pass<ITEMS, 256, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps, gps,
-                                                       shift, width, row_total, table)
-                       : radix_pass<ITEMS, 256, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, bps,
-                                                        gps, shift, width, row_total, table);
+            e = sparse ? radix_pass<ITEMS, 256, true>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, shift,
+                                                       width, row_total, table)
+                       : radix_pass<ITEMS, 256, false>(s, keys, vals, cur, seg_counts, nseg, seg_stride, n_cap, shift,
+                                                        width, row_total, table);
         if (e != hipSuccess) return e;
         shift += width;
         cur = 1 - cur;

@@ -30,9 +30,12 @@ constexpr int BLEND_THREADS = 256;    // one pixel of a 16x16 tile per thread
 constexpr int RADIX_THREADS = 256;    // one or two digits per thread
 // Keys per thread of a radix tile: 16 (4096-key tiles) for large sorts, 4
 // (1024-key tiles) for sorts whose segments hold at most RADIX_SMALL_MAX keys
-// together, whose 4096-key tiles would leave most CUs idle.
+// together, whose 4096-key tiles would leave most CUs idle. Their 512-bin
+// passes take RADIX_ITEMS_WIDE (2048-key tiles): 1024 keys over 512 digits
+// leave runs of about 2 keys, whose scattered writes barely coalesce.
 constexpr int RADIX_ITEMS_LARGE = 16;
 constexpr int RADIX_ITEMS_SMALL = 4;
+constexpr int RADIX_ITEMS_WIDE = 8;
 constexpr uint64_t RADIX_SMALL_MAX = 16ull << 20;
 __host__ __device__ constexpr uint32_t radix_tile_keys(uint32_t nseg, uint64_t n_cap) {
     return (uint32_t)(RADIX_THREADS * ((uint64_t)nseg * n_cap <= RADIX_SMALL_MAX ? RADIX_ITEMS_SMALL : RADIX_ITEMS_LARGE));
